@@ -1,0 +1,258 @@
+"""protoray_hip: Python binding of libprt_hip.so, the MI355X wavefront diffuse
+path tracer behind ProtoRay's Device API (render/device.h:29-54).
+
+The binding is a thin ctypes layer over the C ABI in include/prt_hip.h.  It
+raises RuntimeError with the library's message on every nonzero status, the
+way the reference's Device methods throw (render/device_cuda.cpp:63,92).
+There is no CPU fallback: if the HIP library is missing or no GPU is present,
+constructing HipDevice fails.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+REPO_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "libprt_hip.so")
+HEADER_PATH = os.path.join(REPO_DIR, "include", "prt_hip.h")
+
+RENDERER_PRIMARY = 0
+RENDERER_DIFFUSE = 1
+RENDERER_IDS = {"primaryHip": RENDERER_PRIMARY, "diffuseHip": RENDERER_DIFFUSE}
+
+RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tnear", "<f4"), ("dir", "<f4", 3), ("tfar", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_id", "<i4"), ("u", "<f4"), ("v", "<f4")])
+NODE_BYTES = 80
+
+
+class PrtCamera(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_float * 3), ("image_o", ctypes.c_float * 3),
+                ("image_dx", ctypes.c_float * 3), ("image_dy", ctypes.c_float * 3),
+                ("basis_u", ctypes.c_float * 3), ("basis_v", ctypes.c_float * 3),
+                ("lens_radius", ctypes.c_float), ("focal_distance", ctypes.c_float)]
+
+
+class PrtBuildStats(ctypes.Structure):
+    _fields_ = [("build_ms", ctypes.c_double), ("build_mprim", ctypes.c_double),
+                ("node_count", ctypes.c_int64), ("tri_count", ctypes.c_int64),
+                ("depth", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class PrtRenderStats(ctypes.Structure):
+    _fields_ = [("rays", ctypes.c_int64), ("ms", ctypes.c_double), ("mray", ctypes.c_double),
+                ("paths", ctypes.c_int64), ("queue", ctypes.c_int64 * 16)]
+
+
+class PrtTraceCounters(ctypes.Structure):
+    _fields_ = [("rays", ctypes.c_int64), ("nodes", ctypes.c_int64), ("tris", ctypes.c_int64),
+                ("spills", ctypes.c_int64)]
+
+
+def declared_symbols(header: str = HEADER_PATH) -> list[str]:
+    """Every function name declared in include/prt_hip.h."""
+    text = open(header).read()
+    return sorted(set(re.findall(r"\b(prt_hip_\w+)\s*\(", text)))
+
+
+_LIB = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libprt_hip.so (built in-tree by `make -C embree-benchmark-protoray_amd`)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    i64 = ctypes.c_int64
+    sigs = {
+        "prt_hip_abi_version": ([], ctypes.c_int),
+        "prt_hip_create": ([ctypes.c_int, ctypes.POINTER(P)], ctypes.c_int),
+        "prt_hip_destroy": ([P], ctypes.c_int),
+        "prt_hip_last_error": ([P], ctypes.c_char_p),
+        "prt_hip_device_name": ([P, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+        "prt_hip_scene_upload": ([P, P, i64, P, i64], ctypes.c_int),
+        "prt_hip_bvh_build": ([P, ctypes.POINTER(PrtBuildStats)], ctypes.c_int),
+        "prt_hip_intersect_closest": ([P, P, P, i64], ctypes.c_int),
+        "prt_hip_intersect_closest_device": ([P, P, P, i64], ctypes.c_int),
+        "prt_hip_intersect_counters": ([P, P, P, i64, ctypes.POINTER(PrtTraceCounters)], ctypes.c_int),
+        "prt_hip_camera_from_view": ([ctypes.POINTER(ctypes.c_float * 3), ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_float, ctypes.POINTER(PrtCamera)], ctypes.c_int),
+        "prt_hip_camera_set": ([P, ctypes.POINTER(PrtCamera)], ctypes.c_int),
+        "prt_hip_frame_init": ([P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "prt_hip_frame_clear": ([P], ctypes.c_int),
+        "prt_hip_render": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.POINTER(PrtRenderStats)],
+                           ctypes.c_int),
+        "prt_hip_set_pass": ([P, ctypes.c_int], ctypes.c_int),
+        "prt_hip_accum_read": ([P, P], ctypes.c_int),
+        "prt_hip_frame_read_bgr8": ([P, P], ctypes.c_int),
+        "prt_hip_debug_raygen": ([P, ctypes.c_int, P, P], ctypes.c_int),
+        "prt_hip_bvh_info": ([P, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_int32)],
+                             ctypes.c_int),
+        "prt_hip_bvh_read": ([P, P, P, P], ctypes.c_int),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _LIB = lib
+    return lib
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def camera_struct(cam: np.ndarray) -> PrtCamera:
+    """PrtCamera from 18 floats: origin, imageO, imageDx, imageDy, basisU, basisV."""
+    cam = np.asarray(cam, dtype=np.float32).ravel()
+    c = PrtCamera()
+    for k, name in enumerate(["origin", "image_o", "image_dx", "image_dy", "basis_u", "basis_v"]):
+        if 3 * k + 3 <= cam.size:
+            getattr(c, name)[:] = [float(x) for x in cam[3 * k:3 * k + 3]]
+    c.lens_radius = 0.0
+    c.focal_distance = 1.0
+    return c
+
+
+def camera_from_view(position, angle_x: float, angle_y: float, fov_y: float, aspect: float) -> np.ndarray:
+    """18 floats (origin, imageO, imageDx, imageDy, basisU, basisV) of a view's pinhole camera."""
+    lib = load_library()
+    pos = (ctypes.c_float * 3)(*[float(x) for x in position])
+    cam = PrtCamera()
+    if lib.prt_hip_camera_from_view(ctypes.byref(pos), angle_x, angle_y, fov_y, aspect, ctypes.byref(cam)) != 0:
+        raise RuntimeError("camera_from_view failed")
+    out = []
+    for name in ["origin", "image_o", "image_dx", "image_dy", "basis_u", "basis_v"]:
+        out.extend(getattr(cam, name))
+    return np.array(out, dtype=np.float32)
+
+
+class HipDevice:
+    """One prt_hip_ctx: scene, BVH8, camera, frame and wavefront queues on one GPU."""
+
+    def __init__(self, device_index: int = 0):
+        self.lib = load_library()
+        ctx = ctypes.c_void_p()
+        rc = self.lib.prt_hip_create(device_index, ctypes.byref(ctx))
+        if rc != 0:
+            raise RuntimeError(f"prt_hip_create failed with status {rc} (no usable GPU?)")
+        self.ctx = ctx
+        self.width = self.height = 0
+        self.tri_count = 0
+
+    # -- plumbing
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            msg = self.lib.prt_hip_last_error(self.ctx).decode()
+            raise RuntimeError(f"{what}: {msg}")
+
+    def close(self) -> None:
+        if getattr(self, "ctx", None):
+            self.lib.prt_hip_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> str:
+        buf = ctypes.create_string_buffer(256)
+        self._check(self.lib.prt_hip_device_name(self.ctx, buf, 256), "device_name")
+        return buf.value.decode()
+
+    # -- scene / BVH
+    def upload_scene(self, idx: np.ndarray, pos: np.ndarray) -> None:
+        idx = np.ascontiguousarray(idx, dtype=np.int32).reshape(-1, 3)
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        self._keep = (idx, pos)
+        self._check(self.lib.prt_hip_scene_upload(self.ctx, _ptr(idx), idx.shape[0], _ptr(pos), pos.shape[0]),
+                    "scene_upload")
+        self.tri_count = idx.shape[0]
+
+    def build(self) -> dict:
+        st = PrtBuildStats()
+        self._check(self.lib.prt_hip_bvh_build(self.ctx, ctypes.byref(st)), "bvh_build")
+        return {"buildMs": st.build_ms, "buildMprim": st.build_mprim, "nodes": st.node_count,
+                "tris": st.tri_count, "depth": st.depth}
+
+    def bvh(self) -> tuple[np.ndarray, np.ndarray, np.ndarray, int]:
+        nn, nt, dp = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
+        self._check(self.lib.prt_hip_bvh_info(self.ctx, ctypes.byref(nn), ctypes.byref(nt), ctypes.byref(dp)),
+                    "bvh_info")
+        nodes = np.zeros((nn.value, NODE_BYTES), dtype=np.uint8)
+        tris = np.zeros((nt.value, 9), dtype=np.float32)
+        prims = np.zeros(nt.value, dtype=np.int32)
+        self._check(self.lib.prt_hip_bvh_read(self.ctx, _ptr(nodes), _ptr(tris), _ptr(prims)), "bvh_read")
+        return nodes, tris, prims, dp.value
+
+    # -- closest hit
+    def intersect(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        hits = np.zeros((rays.shape[0], 4), dtype=np.float32)
+        self._check(self.lib.prt_hip_intersect_closest(self.ctx, _ptr(rays), _ptr(hits), rays.shape[0]),
+                    "intersect_closest")
+        return hits
+
+    def intersect_counters(self, rays: np.ndarray) -> tuple[np.ndarray, dict]:
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        hits = np.zeros((rays.shape[0], 4), dtype=np.float32)
+        c = PrtTraceCounters()
+        self._check(self.lib.prt_hip_intersect_counters(self.ctx, _ptr(rays), _ptr(hits), rays.shape[0],
+                                                        ctypes.byref(c)), "intersect_counters")
+        return hits, {"rays": c.rays, "nodes": c.nodes, "tris": c.tris, "spills": c.spills}
+
+    # -- camera / frame / render
+    def set_camera(self, cam: np.ndarray) -> None:
+        c = camera_struct(cam)
+        self._check(self.lib.prt_hip_camera_set(self.ctx, ctypes.byref(c)), "camera_set")
+
+    def init_frame(self, width: int, height: int) -> None:
+        self._check(self.lib.prt_hip_frame_init(self.ctx, width, height), "frame_init")
+        self.width, self.height = width, height
+
+    def clear_frame(self) -> None:
+        self._check(self.lib.prt_hip_frame_clear(self.ctx), "frame_clear")
+
+    def set_pass(self, p: int) -> None:
+        self._check(self.lib.prt_hip_set_pass(self.ctx, p), "set_pass")
+
+    def render(self, renderer: int | str = RENDERER_DIFFUSE, max_depth: int = 6, albedo: float = 0.8) -> dict:
+        if isinstance(renderer, str):
+            if renderer not in RENDERER_IDS:
+                raise ValueError("invalid renderer type")
+            renderer = RENDERER_IDS[renderer]
+        st = PrtRenderStats()
+        self._check(self.lib.prt_hip_render(self.ctx, renderer, max_depth, albedo, ctypes.byref(st)), "render")
+        return {"ray": st.rays, "renderMs": st.ms, "mray": st.mray, "paths": st.paths,
+                "queue": [st.queue[k] for k in range(16)]}
+
+    def accum(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 4), dtype=np.float32)
+        self._check(self.lib.prt_hip_accum_read(self.ctx, _ptr(out)), "accum_read")
+        return out
+
+    def frame_bgr8(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width), dtype=np.int32)
+        self._check(self.lib.prt_hip_frame_read_bgr8(self.ctx, _ptr(out)), "frame_read_bgr8")
+        return out
+
+    def debug_raygen(self, pass_index: int) -> tuple[np.ndarray, np.ndarray]:
+        rays = np.zeros((self.height * self.width, 8), dtype=np.float32)
+        states = np.zeros(self.height * self.width, dtype=np.uint32)
+        self._check(self.lib.prt_hip_debug_raygen(self.ctx, pass_index, _ptr(rays), _ptr(states)), "debug_raygen")
+        return rays, states

@@ -1,0 +1,138 @@
+// prt_internal.h -- device data layout and the context behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "prt_math.h"
+#include "../../include/prt_hip.h"
+
+namespace prt {
+
+// BVH8 node, 80 B, quantised child boxes (S. Ylitie, T. Karras, S. Laine,
+// "Efficient Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs",
+// HPG 2017).  Child k's box is p + 2^e * [qlo_k, qhi_k] per axis.  Internal
+// children are stored contiguously from child_base in slot order (their index
+// is child_base + popcount(imask below k)); a leaf slot k has meta[k] =
+// count << 5 | offset and owns triangle records tri_base + offset ..
+// + count - 1.  An empty slot has meta 0, imask bit 0 and an inverted box.
+struct alignas(16) Bvh8Node {
+    float px, py, pz;
+    uint8_t ex, ey, ez;   // int8 exponents stored as two's complement bytes
+    uint8_t imask;
+    uint32_t child_base;
+    uint32_t tri_base;
+    uint8_t meta[8];
+    uint8_t qlo_x[8], qlo_y[8], qlo_z[8];
+    uint8_t qhi_x[8], qhi_y[8], qhi_z[8];
+};
+static_assert(sizeof(Bvh8Node) == 80, "BVH8 node must be 80 bytes");
+
+constexpr int kLeafMax = 3;           // triangles per leaf slot
+constexpr int kTraceBlock = 256;      // threads per trace workgroup
+constexpr int kMaxQueueDepth = 16;    // max_depth + 1 <= 16
+
+// Triangle record: the three fp32 vertices (36 B), in BVH8 leaf order.
+struct TriRecord {
+    float v[9];
+};
+
+struct SceneDevice {
+    const Bvh8Node* nodes;
+    const float* tris;        // 9 floats per record
+    const int32_t* prim_ids;  // original triangle index per record
+    int64_t tri_count;
+};
+
+}  // namespace prt
+
+struct prt_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int num_cus = 0;
+    std::string device_name;
+
+    // scene
+    int64_t tri_count = 0, vertex_count = 0;
+    int32_t* d_idx = nullptr;
+    float* d_pos = nullptr;
+
+    // BVH8
+    prt::Bvh8Node* d_nodes = nullptr;
+    float* d_tris = nullptr;
+    int32_t* d_prim_ids = nullptr;
+    int64_t node_count = 0;
+    int32_t bvh_depth = 0;
+    bool bvh_ready = false;
+
+    // traversal spill stack (global part of the short stack)
+    uint2* d_spill = nullptr;
+    int64_t spill_bytes = 0;
+    int trace_grid = 0;        // persistent workgroups per trace launch
+    int stack_entries = 0;     // max stack depth per lane (LDS + spill)
+
+    // frame
+    int width = 0, height = 0;
+    float4* d_accum = nullptr;
+    int32_t* d_bgr8 = nullptr;
+    int pass = 0;
+    prt_camera cam{};
+    bool cam_ready = false;
+
+    // wavefront queues
+    int64_t queue_cap = 0;
+    float4* d_rays[2] = {nullptr, nullptr};   // 2 float4 per ray
+    float4* d_hits = nullptr;                 // 1 float4 per hit
+    int32_t* d_pix[2] = {nullptr, nullptr};
+    uint32_t* d_rng[2] = {nullptr, nullptr};
+    uint32_t* d_counters = nullptr;           // [0..15] queue sizes, [16..31] fetch, [32] error
+    unsigned long long* d_stats = nullptr;    // instrumented counters
+
+    // host-array intersect staging
+    float4* d_io_rays = nullptr;
+    float4* d_io_hits = nullptr;
+    int64_t io_cap = 0;
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace prt {
+
+// bvh_build.hip
+int build_bvh8(prt_hip_ctx* ctx, prt_build_stats* stats);
+
+// trace.hip
+struct TraceLaunch {
+    const float4* rays;
+    float4* hits;
+    const uint32_t* count_ptr;  // device count of rays (nullptr: use count)
+    uint32_t count;
+    uint32_t* fetch_ptr;        // device fetch counter, zero before launch
+    unsigned long long* stats;  // nullptr unless instrumented
+};
+int ensure_trace_resources(prt_hip_ctx* ctx);
+hipError_t launch_trace(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumented);
+
+// render.hip
+hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t s);
+hipError_t launch_raygen(prt_hip_ctx* ctx, int pass, float4* rays, int32_t* pix, uint32_t* rng);
+hipError_t launch_shade_diffuse(prt_hip_ctx* ctx, int depth, int max_depth, float throughput,
+                                float albedo_unused);
+hipError_t launch_shade_primary(prt_hip_ctx* ctx);
+hipError_t launch_tonemap(prt_hip_ctx* ctx);
+hipError_t launch_unpack_rays(const float4* rays, const int32_t* pix, const uint32_t* rng,
+                              int64_t n, int width, int height, prt_ray* out_rays,
+                              uint32_t* out_states, hipStream_t s);
+
+int set_error(prt_hip_ctx* ctx, const std::string& msg);
+int check_hip(prt_hip_ctx* ctx, hipError_t e, const char* what);
+
+}  // namespace prt
+
+#define PRT_TRY(ctx, expr)                                              \
+    do {                                                                \
+        hipError_t _e = (expr);                                         \
+        if (_e != hipSuccess) return prt::check_hip((ctx), _e, #expr);  \
+    } while (0)

@@ -1,0 +1,284 @@
+// prt_math.h -- scalar numerics of the diffuse path, shared by every kernel.
+//
+// All functions here are written so that their fp32 results do not depend on
+// the compiler: floating-point contraction is off, fused multiply-adds are
+// spelled out with fmaf, and only correctly rounded IEEE operations are used
+// (add, sub, mul, div, sqrt, fma).  A CPU implementation that performs the
+// same operations in the same order therefore reproduces every bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PRT_HD __host__ __device__ __forceinline__
+
+#pragma clang fp contract(off)
+
+namespace prt {
+
+struct f3 {
+    float x, y, z;
+};
+
+PRT_HD f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+PRT_HD f3 add3(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+PRT_HD f3 sub3(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+PRT_HD f3 mul3(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+PRT_HD f3 neg3(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+PRT_HD float dot3(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+PRT_HD f3 cross3(f3 a, f3 b)
+{
+    return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// normalize(a) = a * (1 / sqrt(a.a)); the reference uses an approximate
+// rsqrt (math/math_avx.h:30-34, math/vec3.h:395-398), this build the exact one.
+PRT_HD f3 normalize3(f3 a)
+{
+    float inv = 1.0f / sqrtf(dot3(a, a));
+    return mul3(a, inv);
+}
+PRT_HD float max3f(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
+
+// ---------------------------------------------------------------------------
+// Sampler: murmur3 x86_32 mix/finalize of (seed=0, pixelId, pass), then an LCG
+// per 1D draw.  sampling/random_sampler.h:44-73, math/hash.h:23-72.
+// ---------------------------------------------------------------------------
+PRT_HD uint32_t rotl32(uint32_t x, uint32_t r) { return (x << r) | (x >> (32u - r)); }
+
+PRT_HD uint32_t murmur_mix(uint32_t h, uint32_t k)
+{
+    k *= 0xcc9e2d51u;
+    k = rotl32(k, 15u);
+    k *= 0x1b873593u;
+    h ^= k;
+    h = rotl32(h, 13u) * 5u + 0xe6546b64u;
+    return h;
+}
+
+PRT_HD uint32_t murmur_fmix(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+PRT_HD uint32_t sampler_seed(int pass, int pixel_id)
+{
+    uint32_t h = 0u;
+    h = murmur_mix(h, (uint32_t)pixel_id);
+    h = murmur_mix(h, (uint32_t)pass);
+    return murmur_fmix(h);
+}
+
+PRT_HD uint32_t lcg_next(uint32_t s) { return s * 1664525u + 1013904223u; }
+
+// u32 -> [0,1]: round-to-nearest conversion times 2^-32, the CPU scalar form
+// (math/math.h:179).  It can return exactly 1.0f.
+PRT_HD float to_unorm(uint32_t s) { return (float)s * 0x1.0p-32f; }
+
+PRT_HD float sampler_next(uint32_t& s)
+{
+    s = lcg_next(s);
+    return to_unorm(s);
+}
+
+// ---------------------------------------------------------------------------
+// sin(2 pi v), cos(2 pi v) for v in [0, 1]: quadrant reduction by k = round(4v)
+// (exact: 4v is a power-of-two scaling and 4v - k is exact by Sterbenz), then
+// Taylor polynomials of sin and cos on |x| <= pi/4 evaluated by Horner with
+// fmaf.  Truncation error < 3e-8 absolute, below fp32 rounding at this range.
+// ---------------------------------------------------------------------------
+PRT_HD void sincos_2pi(float v, float& s_out, float& c_out)
+{
+    float w = v * 4.0f;
+    int k = (int)(w + 0.5f);
+    float f = w - (float)k;
+    float x = f * 1.57079632679489662f;  // pi/2
+    float x2 = x * x;
+    // sin x = x (1 - x^2/3! + x^4/5! - x^6/7! + x^8/9!)
+    float ps = 2.75573192239858907e-6f;          // 1/9!
+    ps = fmaf(ps, x2, -1.98412698412698413e-4f); // -1/7!
+    ps = fmaf(ps, x2, 8.33333333333333333e-3f);  // 1/5!
+    ps = fmaf(ps, x2, -1.66666666666666667e-1f); // -1/3!
+    float sx = fmaf(ps * x2, x, x);
+    // cos x = 1 - x^2/2! + x^4/4! - x^6/6! + x^8/8! - x^10/10!
+    float pc = -2.75573192239858907e-7f;         // -1/10!
+    pc = fmaf(pc, x2, 2.48015873015873016e-5f);  // 1/8!
+    pc = fmaf(pc, x2, -1.38888888888888889e-3f); // -1/6!
+    pc = fmaf(pc, x2, 4.16666666666666667e-2f);  // 1/4!
+    pc = fmaf(pc, x2, -0.5f);
+    float cx = fmaf(pc, x2, 1.0f);
+    switch (k & 3) {
+    case 0: s_out = sx; c_out = cx; break;
+    case 1: s_out = cx; c_out = -sx; break;
+    case 2: s_out = -sx; c_out = -cx; break;
+    default: s_out = -cx; c_out = sx; break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pinhole camera ray (camera/pinhole_camera.h:51-55): image = (pixel + jitter)
+// * invSize; dir = normalize(imageO + image.x imageDx + image.y imageDy).
+// ---------------------------------------------------------------------------
+struct Camera {
+    f3 origin, image_o, image_dx, image_dy;
+};
+
+PRT_HD f3 camera_dir(const Camera& c, float ix, float iy)
+{
+    f3 d;
+    d.x = (c.image_o.x + ix * c.image_dx.x) + iy * c.image_dy.x;
+    d.y = (c.image_o.y + ix * c.image_dx.y) + iy * c.image_dy.y;
+    d.z = (c.image_o.z + ix * c.image_dx.z) + iy * c.image_dy.z;
+    return normalize3(d);
+}
+
+// ---------------------------------------------------------------------------
+// Diffuse bounce (render/diffuse_integrator_single.h:55-62 with
+// geometry/triangle_mesh.cpp:248-273, core/ray.h:47-52,71-76,
+// math/basis3.h:128-145, sampling/shape_sampler.h:50-74).
+// ---------------------------------------------------------------------------
+// Geometric normal of the hit triangle, flipped to face against the ray.
+PRT_HD f3 face_normal(f3 p0, f3 p1, f3 p2, f3 dir)
+{
+    f3 ng = normalize3(cross3(sub3(p1, p0), sub3(p2, p0)));
+    if (dot3(ng, dir) > 0.0f) ng = neg3(ng);
+    return ng;
+}
+
+// New ray of a diffuse bounce at hit distance t along (org, dir): cosine-
+// weighted direction around n, origin offset by eps along it, tnear 0.
+PRT_HD void diffuse_bounce(f3 org, f3 dir, float t, f3 n, float s1, float s2, f3& org_o, f3& dir_o)
+{
+    f3 p;
+    p.x = org.x + t * dir.x;
+    p.y = org.y + t * dir.y;
+    p.z = org.z + t * dir.z;
+    float eps = fmaxf(t, max3f(fabsf(p.x), fabsf(p.y), fabsf(p.z))) * 0x1.fp-18f;
+    // makeFrame
+    f3 u0 = mk3(0.0f, n.z, -n.y);
+    f3 u1 = mk3(-n.z, 0.0f, n.x);
+    f3 u = normalize3(dot3(u0, u0) > dot3(u1, u1) ? u0 : u1);
+    f3 v = normalize3(cross3(n, u));
+    // cosineSampleHemisphere
+    float cos_t = sqrtf(s1);
+    float sin_t = sqrtf(1.0f - s1);
+    float sp, cp;
+    sincos_2pi(s2, sp, cp);
+    float lx = cp * sin_t;
+    float ly = sp * sin_t;
+    float lz = cos_t;
+    f3 d;
+    d.x = (lx * u.x + ly * v.x) + lz * n.x;
+    d.y = (lx * u.y + ly * v.y) + lz * n.y;
+    d.z = (lx * u.z + ly * v.z) + lz * n.z;
+    dir_o = d;
+    org_o.x = p.x + eps * d.x;
+    org_o.y = p.y + eps * d.y;
+    org_o.z = p.z + eps * d.z;
+}
+
+// ---------------------------------------------------------------------------
+// Watertight ray/triangle test (S. Woop, C. Benthin, I. Wald, "Watertight
+// Ray/Triangle Intersection", JCGT 2(1), 2013).  Per-ray set-up below; the
+// per-triangle part is tri_test.  U, V, W are the scaled barycentrics of
+// p0, p1, p2; a zero edge function is recomputed in double.
+// ---------------------------------------------------------------------------
+struct WatertightRay {
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+
+PRT_HD float comp(f3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+PRT_HD WatertightRay watertight_setup(f3 d)
+{
+    WatertightRay w;
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = 0;
+    float m = ax;
+    if (ay > m) { kz = 1; m = ay; }
+    if (az > m) { kz = 2; }
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    float dz = comp(d, kz);
+    if (dz < 0.0f) { int t = kx; kx = ky; ky = t; }
+    w.kx = kx; w.ky = ky; w.kz = kz;
+    w.sx = comp(d, kx) / dz;
+    w.sy = comp(d, ky) / dz;
+    w.sz = 1.0f / dz;
+    return w;
+}
+
+// Candidate intersection.  Returns false for a miss or for a hit that lies
+// outside [tmin, tbest*(1+2^-20)]; otherwise t = T/det, and (U,V,W,det) are
+// kept so that the caller computes u = V/det, v = W/det only for accepted hits.
+struct TriCandidate {
+    float t, v_num, w_num, det;
+};
+
+PRT_HD bool tri_test(const WatertightRay& w, f3 org, f3 p0, f3 p1, f3 p2, float tmin, float tbest,
+                     TriCandidate& c)
+{
+    f3 a = sub3(p0, org), b = sub3(p1, org), cc = sub3(p2, org);
+    float az_ = comp(a, w.kz), bz_ = comp(b, w.kz), cz_ = comp(cc, w.kz);
+    float ax = comp(a, w.kx) - w.sx * az_;
+    float ay = comp(a, w.ky) - w.sy * az_;
+    float bx = comp(b, w.kx) - w.sx * bz_;
+    float by = comp(b, w.ky) - w.sy * bz_;
+    float cx = comp(cc, w.kx) - w.sx * cz_;
+    float cy = comp(cc, w.ky) - w.sy * cz_;
+    float U = cx * by - cy * bx;
+    float V = ax * cy - ay * cx;
+    float W = bx * ay - by * ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)cx * (double)by - (double)cy * (double)bx);
+        V = (float)((double)ax * (double)cy - (double)ay * (double)cx);
+        W = (float)((double)bx * (double)ay - (double)by * (double)ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = (U + V) + W;
+    if (det == 0.0f) return false;
+    float azs = w.sz * az_, bzs = w.sz * bz_, czs = w.sz * cz_;
+    float T = (U * azs + V * bzs) + W * czs;
+    float tdet = det < 0.0f ? -T : T;
+    float adet = fabsf(det);
+    if (tdet < tmin * adet) return false;
+    if (tdet > (tbest * adet) * 1.00000095367431641f) return false;  // 1 + 2^-20
+    float t = T / det;
+    if (!(t >= tmin)) return false;
+    c.t = t;
+    c.v_num = V;
+    c.w_num = W;
+    c.det = det;
+    return true;
+}
+
+// Distance to the winning triangle's plane, in double:
+// t = ((p0 - o) . n) / (d . n) with n = (p1 - p0) x (p2 - p0), rounded to
+// float.  Reported instead of the watertight test's fp32 T/det, whose error
+// grows at grazing angles and near the ray origin; the choice of triangle is
+// the watertight test's.  Falls back to t32 if the plane is parallel to the ray.
+PRT_HD float refine_t(f3 p0, f3 p1, f3 p2, f3 o, f3 d, float t32)
+{
+    double e1x = (double)p1.x - (double)p0.x, e1y = (double)p1.y - (double)p0.y,
+           e1z = (double)p1.z - (double)p0.z;
+    double e2x = (double)p2.x - (double)p0.x, e2y = (double)p2.y - (double)p0.y,
+           e2z = (double)p2.z - (double)p0.z;
+    double nx = e1y * e2z - e1z * e2y;
+    double ny = e1z * e2x - e1x * e2z;
+    double nz = e1x * e2y - e1y * e2x;
+    double wx = (double)p0.x - (double)o.x, wy = (double)p0.y - (double)o.y,
+           wz = (double)p0.z - (double)o.z;
+    double num = (wx * nx + wy * ny) + wz * nz;
+    double den = ((double)d.x * nx + (double)d.y * ny) + (double)d.z * nz;
+    if (den == 0.0) return t32;
+    double t = num / den;
+    if (!(t >= 0.0)) return t32;
+    return (float)t;
+}
+
+}  // namespace prt

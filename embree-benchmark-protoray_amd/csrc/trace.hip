@@ -1,0 +1,315 @@
+// trace.hip -- closest-hit traversal of the quantised BVH8.
+//
+// Replaces IntersectorStreamCuda::intersect (core/intersector_stream_cuda.h:27-34),
+// which the reference forwards to OptiX Prime's closest query
+// (accel/optix/optix_intersector_stream_cuda.cpp:77-82).
+//
+// One ray per lane.  The kernel is persistent: each wave takes rays from a
+// global counter; when at least kRefill of its 64 lanes have finished, the
+// wave ballots the idle lanes, takes that many new rays with one atomic and
+// hands them out by lane prefix (mbcnt), so lanes stay busy on incoherent
+// bounces.  The traversal stack is a per-lane ring of kStackLds entries in
+// LDS; deeper entries spill to a per-lane slice of global memory.  Each entry
+// keeps the entry distance of its box, so nodes that fall behind the closest
+// hit are dropped when popped.
+//
+// Result: the hit with the smallest (t, primId) among triangles whose
+// watertight test accepts tnear <= t <= tfar; the same answer as testing every
+// triangle, independent of the visiting order.
+#include "prt_internal.h"
+
+namespace prt {
+namespace {
+
+constexpr int kStackLds = 16;      // LDS entries per lane (power of two)
+constexpr int kRefill = 24;        // refill when this many lanes are idle
+
+__device__ __forceinline__ float byte_f(uint32_t w, int k)
+{
+    return (float)((w >> (8 * k)) & 0xffu);
+}
+
+__device__ __forceinline__ float as_f(uint32_t u) { return __uint_as_float(u); }
+
+template <bool kCount>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaunch tl, uint2* spill,
+                                                       int stack_entries, uint32_t* err)
+{
+    __shared__ uint32_t st_node[kStackLds * kTraceBlock];
+    __shared__ float st_t[kStackLds * kTraceBlock];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const uint64_t lane_lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t count = tl.count_ptr ? *tl.count_ptr : tl.count;
+    uint2* my_spill = spill + ((size_t)blockIdx.x * kTraceBlock + tid) * (size_t)stack_entries;
+    const int spill_cap = stack_entries;
+
+    bool active = false;
+    uint32_t ray_idx = 0;
+    f3 o{0, 0, 0}, d{0, 0, 0};
+    float tmin = 0.0f, tbest = 0.0f;
+    float inv_x = 0, inv_y = 0, inv_z = 0;
+    bool neg_x = false, neg_y = false, neg_z = false;
+    WatertightRay wr{};
+    int32_t best_prim = -1;
+    uint32_t best_rec = 0;
+    float bu = 0.0f, bv = 0.0f;
+    int sp = 0, bottom = 0;
+    int32_t node = -1;
+    unsigned long long n_nodes = 0, n_tris = 0, n_spill = 0;
+    bool exhausted = false;  // wave-uniform: the ray counter has run past count
+
+    for (;;) {
+        // ---- refill idle lanes (wave-uniform branch)
+        uint64_t idle = __ballot(!active);
+        int n_idle = __popcll(idle);
+        if (!exhausted && n_idle >= kRefill) {
+            int leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(tl.fetch_ptr, (uint32_t)n_idle);
+            base = __shfl(base, leader, 64);
+            if (!active) {
+                uint32_t my = base + (uint32_t)__popcll(idle & lane_lt);
+                if (my < count) {
+                    ray_idx = my;
+                    float4 r0 = tl.rays[2 * (size_t)my];
+                    float4 r1 = tl.rays[2 * (size_t)my + 1];
+                    o = mk3(r0.x, r0.y, r0.z);
+                    tmin = r0.w;
+                    d = mk3(r1.x, r1.y, r1.z);
+                    tbest = r1.w;
+                    if (tbest < 0.0f) {
+                        tl.hits[my] = make_float4(-1.0f, __int_as_float(-1), 0.0f, 0.0f);
+                    } else {
+                        active = true;
+                        float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
+                        float dy = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
+                        float dz = fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z;
+                        inv_x = __builtin_amdgcn_rcpf(dx);
+                        inv_y = __builtin_amdgcn_rcpf(dy);
+                        inv_z = __builtin_amdgcn_rcpf(dz);
+                        neg_x = dx < 0.0f;
+                        neg_y = dy < 0.0f;
+                        neg_z = dz < 0.0f;
+                        wr = watertight_setup(d);
+                        best_prim = -1;
+                        bu = bv = 0.0f;
+                        sp = bottom = 0;
+                        node = 0;
+                    }
+                }
+            }
+            if ((uint64_t)base + (uint64_t)n_idle >= count) exhausted = true;
+        }
+        if (__ballot(active) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+
+        // ---- traverse until too many lanes of the wave are idle
+        while (active) {
+            if (node < 0) {
+                // pop the next node that can still hold a closer hit
+                while (sp > 0) {
+                    --sp;
+                    uint32_t nid;
+                    float nt;
+                    if (sp < bottom) {
+                        uint2 e = my_spill[sp];
+                        bottom = sp;
+                        nid = e.x;
+                        nt = as_f(e.y);
+                    } else {
+                        int slot = (sp & (kStackLds - 1)) * kTraceBlock + tid;
+                        nid = st_node[slot];
+                        nt = st_t[slot];
+                    }
+                    if (nt <= tbest) { node = (int32_t)nid; break; }
+                }
+                if (node < 0) {
+                    if (best_prim >= 0) {
+                        const float* tp = sc.tris + 9 * (size_t)best_rec;
+                        const float t = refine_t(mk3(tp[0], tp[1], tp[2]), mk3(tp[3], tp[4], tp[5]),
+                                                 mk3(tp[6], tp[7], tp[8]), o, d, tbest);
+                        tl.hits[ray_idx] = make_float4(t, __int_as_float(best_prim), bu, bv);
+                    } else
+                        tl.hits[ray_idx] = make_float4(-1.0f, __int_as_float(-1), 0.0f, 0.0f);
+                    active = false;
+                    break;
+                }
+            }
+
+            // ---- visit one BVH8 node
+            const uint4* np = reinterpret_cast<const uint4*>(sc.nodes + node);
+            uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
+            if (kCount) ++n_nodes;
+            const int ex = (int)(int8_t)(w0.w & 0xffu);
+            const int ey = (int)(int8_t)((w0.w >> 8) & 0xffu);
+            const int ez = (int)(int8_t)((w0.w >> 16) & 0xffu);
+            const uint32_t imask = w0.w >> 24;
+            const float ax = __uint_as_float((uint32_t)(ex + 127) << 23) * inv_x;
+            const float ay = __uint_as_float((uint32_t)(ey + 127) << 23) * inv_y;
+            const float az = __uint_as_float((uint32_t)(ez + 127) << 23) * inv_z;
+            const float bx = (as_f(w0.x) - o.x) * inv_x;
+            const float by = (as_f(w0.y) - o.y) * inv_y;
+            const float bz = (as_f(w0.z) - o.z) * inv_z;
+            // rounding slack of fma(q, a, b): 2^-21 (255|a| + |b|)
+            const float slx = 4.76837158203125e-7f * (255.0f * fabsf(ax) + fabsf(bx));
+            const float sly = 4.76837158203125e-7f * (255.0f * fabsf(ay) + fabsf(by));
+            const float slz = 4.76837158203125e-7f * (255.0f * fabsf(az) + fabsf(bz));
+            const float bxn = bx - slx, bxf = bx + slx;
+            const float byn = by - sly, byf = by + sly;
+            const float bzn = bz - slz, bzf = bz + slz;
+            // near/far planes per axis from the direction signs
+            const uint32_t nx0 = neg_x ? w3.z : w2.x, nx1 = neg_x ? w3.w : w2.y;
+            const uint32_t fx0 = neg_x ? w2.x : w3.z, fx1 = neg_x ? w2.y : w3.w;
+            const uint32_t ny0 = neg_y ? w4.x : w2.z, ny1 = neg_y ? w4.y : w2.w;
+            const uint32_t fy0 = neg_y ? w2.z : w4.x, fy1 = neg_y ? w2.w : w4.y;
+            const uint32_t nz0 = neg_z ? w4.z : w3.x, nz1 = neg_z ? w4.w : w3.y;
+            const uint32_t fz0 = neg_z ? w3.x : w4.z, fz1 = neg_z ? w3.y : w4.w;
+            const uint32_t meta0 = w1.z, meta1 = w1.w;
+
+            uint32_t hit_int = 0, hit_leaf = 0;
+            float tk[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int kb = k & 3;
+                const uint32_t nxw = k < 4 ? nx0 : nx1, fxw = k < 4 ? fx0 : fx1;
+                const uint32_t nyw = k < 4 ? ny0 : ny1, fyw = k < 4 ? fy0 : fy1;
+                const uint32_t nzw = k < 4 ? nz0 : nz1, fzw = k < 4 ? fz0 : fz1;
+                float tnx = __builtin_fmaf(byte_f(nxw, kb), ax, bxn);
+                float tny = __builtin_fmaf(byte_f(nyw, kb), ay, byn);
+                float tnz = __builtin_fmaf(byte_f(nzw, kb), az, bzn);
+                float tfx = __builtin_fmaf(byte_f(fxw, kb), ax, bxf);
+                float tfy = __builtin_fmaf(byte_f(fyw, kb), ay, byf);
+                float tfz = __builtin_fmaf(byte_f(fzw, kb), az, bzf);
+                float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
+                float tf = fminf(fminf(fminf(tfx, tfy), tfz), tbest);
+                tk[k] = tn;
+                const uint32_t mb = ((k < 4 ? meta0 : meta1) >> (8 * kb)) & 0xffu;
+                if (tn <= tf) {
+                    if ((imask >> k) & 1u) hit_int |= 1u << k;
+                    else if (mb != 0u) hit_leaf |= 1u << k;
+                }
+            }
+
+            // ---- leaf slots: test their triangles
+            const uint32_t tri_base = w1.y;
+            while (hit_leaf) {
+                const int k = __builtin_ctz(hit_leaf);
+                hit_leaf &= hit_leaf - 1;
+                const uint32_t mb = ((k < 4 ? meta0 : meta1) >> (8 * (k & 3))) & 0xffu;
+                const uint32_t first = tri_base + (mb & 31u);
+                const uint32_t cnt = mb >> 5;
+                for (uint32_t j = 0; j < cnt; ++j) {
+                    const uint32_t rec = first + j;
+                    const float* tp = sc.tris + 9 * (size_t)rec;
+                    f3 p0 = mk3(tp[0], tp[1], tp[2]);
+                    f3 p1 = mk3(tp[3], tp[4], tp[5]);
+                    f3 p2 = mk3(tp[6], tp[7], tp[8]);
+                    if (kCount) ++n_tris;
+                    TriCandidate c;
+                    if (tri_test(wr, o, p0, p1, p2, tmin, tbest, c)) {
+                        if (c.t <= tbest) {
+                            int32_t prim = sc.prim_ids[rec];
+                            if (c.t < tbest || best_prim < 0 || prim < best_prim) {
+                                tbest = c.t;
+                                best_prim = prim;
+                                best_rec = rec;
+                                bu = c.v_num / c.det;
+                                bv = c.w_num / c.det;
+                            }
+                        }
+                    }
+                }
+            }
+
+            // ---- internal slots: go to the nearest, push the others
+            int32_t next = -1;
+            float next_t = 0.0f;
+            const uint32_t child_base = w1.x;
+            while (hit_int) {
+                const int k = __builtin_ctz(hit_int);
+                hit_int &= hit_int - 1;
+                const float t = tk[k];
+                if (t > tbest) continue;
+                const int32_t cidx = (int32_t)(child_base + __popc(imask & ((1u << k) - 1u)));
+                int32_t push_n;
+                float push_t;
+                if (next < 0) { next = cidx; next_t = t; continue; }
+                if (t < next_t) {
+                    push_n = next; push_t = next_t;
+                    next = cidx; next_t = t;
+                } else {
+                    push_n = cidx; push_t = t;
+                }
+                if (sp - bottom == kStackLds) {
+                    if (bottom >= spill_cap) { atomicOr(err, 1u); break; }
+                    int bslot = (bottom & (kStackLds - 1)) * kTraceBlock + tid;
+                    my_spill[bottom] = make_uint2(st_node[bslot], __float_as_uint(st_t[bslot]));
+                    ++bottom;
+                    if (kCount) ++n_spill;
+                }
+                int slot = (sp & (kStackLds - 1)) * kTraceBlock + tid;
+                st_node[slot] = (uint32_t)push_n;
+                st_t[slot] = push_t;
+                ++sp;
+            }
+            node = next;
+
+            if (!exhausted && __popcll(__ballot(1)) <= 64 - kRefill) break;
+        }
+    }
+
+    if (kCount) {
+        for (int off = 32; off > 0; off >>= 1) {
+            n_nodes += __shfl_down(n_nodes, off, 64);
+            n_tris += __shfl_down(n_tris, off, 64);
+            n_spill += __shfl_down(n_spill, off, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(&tl.stats[0], n_nodes);
+            atomicAdd(&tl.stats[1], n_tris);
+            atomicAdd(&tl.stats[2], n_spill);
+        }
+    }
+}
+
+}  // namespace
+
+int ensure_trace_resources(prt_hip_ctx* ctx)
+{
+    if (ctx->trace_grid == 0) {
+        int per_cu = 0;
+        PRT_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false>, kTraceBlock, 0));
+        if (per_cu < 1) per_cu = 1;
+        ctx->trace_grid = per_cu * ctx->num_cus;
+    }
+    // worst case stack: 7 pushes per level of the tree, plus slack
+    int need = 7 * (ctx->bvh_depth + 1) + 8;
+    if (need < kStackLds) need = kStackLds;
+    int64_t bytes = (int64_t)ctx->trace_grid * kTraceBlock * need * (int64_t)sizeof(uint2);
+    if (bytes > ctx->spill_bytes || need != ctx->stack_entries) {
+        if (ctx->d_spill) { (void)hipFree(ctx->d_spill); ctx->d_spill = nullptr; }
+        PRT_TRY(ctx, hipMalloc(&ctx->d_spill, bytes));
+        ctx->spill_bytes = bytes;
+        ctx->stack_entries = need;
+    }
+    return 0;
+}
+
+hipError_t launch_trace(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumented)
+{
+    SceneDevice sc{ctx->d_nodes, ctx->d_tris, ctx->d_prim_ids, ctx->tri_count};
+    uint32_t* err = ctx->d_counters + 32;
+    if (instrumented)
+        k_trace<true><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(sc, t, ctx->d_spill,
+                                                                        ctx->stack_entries, err);
+    else
+        k_trace<false><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(sc, t, ctx->d_spill,
+                                                                         ctx->stack_entries, err);
+    return hipGetLastError();
+}
+
+}  // namespace prt

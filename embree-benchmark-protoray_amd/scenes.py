@@ -1,0 +1,269 @@
+"""Procedural workloads and ProtoRay's file formats.
+
+Formats (byte-for-byte the reference's):
+  .mesh  blob id 0x04df8666 (geometry/triangle_mesh.h:34), then Box3f bounds,
+         int T, int V, bool hasNormals, bool hasTexcoords, int32[3T] indices,
+         int32[T] materialIds, float[V*stride] vertex attributes, and the
+         material names as int count + (int length + bytes) each
+         (geometry/triangle_mesh.cpp:305-337, sys/blob.h:26-50,
+         sys/array.h:357-393, sys/string.cpp:22-45).
+  .view  ViewSet: 10 x View{Vec3f pos; float angleX, angleY, fovY, radius,
+         focus} = 320 bytes, raw (main/render/view.h:25-40, main/render/view.cpp:23-46).
+  .ppm   P6 with the B, G, R bytes of the packed pixel in that order
+         (image/image_io.cpp:50-80).
+
+Workloads: the generators below build the synthetic scenes the benchmark and
+the tests run on.  Their seeds and camera views are fixed in CONFIGS and never
+change once measured.
+"""
+from __future__ import annotations
+
+import math
+import struct
+from dataclasses import dataclass, field
+
+import numpy as np
+
+MESH_BLOB_ID = 0x04DF8666
+
+
+# ------------------------------------------------------------------ formats
+@dataclass
+class Mesh:
+    idx: np.ndarray                      # (T, 3) int32
+    pos: np.ndarray                      # (V, 3) float32
+    material_ids: np.ndarray | None = None
+    material_names: list = field(default_factory=list)
+    normals: np.ndarray | None = None    # (V, 3) float32, optional
+    texcoords: np.ndarray | None = None  # (V, 2) float32, optional
+
+    @property
+    def tri_count(self) -> int:
+        return int(self.idx.shape[0])
+
+    def bounds(self) -> np.ndarray:
+        if self.pos.shape[0] == 0:
+            return np.array([np.inf] * 3 + [-np.inf] * 3, dtype=np.float32)
+        return np.concatenate([self.pos.min(axis=0), self.pos.max(axis=0)]).astype(np.float32)
+
+
+def mesh_to_bytes(m: Mesh) -> bytes:
+    T, V = m.idx.shape[0], m.pos.shape[0]
+    has_n, has_uv = m.normals is not None, m.texcoords is not None
+    attribs = [m.pos.astype(np.float32)]
+    if has_n:
+        attribs.append(m.normals.astype(np.float32))
+    if has_uv:
+        attribs.append(m.texcoords.astype(np.float32))
+    verts = np.concatenate(attribs, axis=1) if len(attribs) > 1 else attribs[0]
+    mat = m.material_ids if m.material_ids is not None else np.zeros(T, dtype=np.int32)
+    names = m.material_names if m.material_names else ["default"]
+    out = [struct.pack("<i", MESH_BLOB_ID), m.bounds().astype("<f4").tobytes(),
+           struct.pack("<iiBB", T, V, int(has_n), int(has_uv)),
+           np.ascontiguousarray(m.idx, dtype="<i4").tobytes(),
+           np.ascontiguousarray(mat, dtype="<i4").tobytes(),
+           np.ascontiguousarray(verts, dtype="<f4").tobytes(),
+           struct.pack("<i", len(names))]
+    for nm in names:
+        b = nm.encode()
+        out.append(struct.pack("<i", len(b)) + b)
+    return b"".join(out)
+
+
+def mesh_from_bytes(data: bytes) -> Mesh:
+    off = 0
+
+    def take(n: int) -> bytes:
+        nonlocal off
+        if off + n > len(data):
+            raise RuntimeError("unexpected end of stream")
+        b = data[off:off + n]
+        off += n
+        return b
+
+    (blob,) = struct.unpack("<i", take(4))
+    if blob != MESH_BLOB_ID:
+        raise RuntimeError("file has wrong blob ID")
+    take(24)  # bounds
+    T, V, has_n, has_uv = struct.unpack("<iiBB", take(10))
+    stride = 3 + (3 if has_n else 0) + (2 if has_uv else 0)
+    idx = np.frombuffer(take(12 * T), dtype="<i4").reshape(T, 3).astype(np.int32)
+    mat = np.frombuffer(take(4 * T), dtype="<i4").astype(np.int32)
+    verts = np.frombuffer(take(4 * V * stride), dtype="<f4").reshape(V, stride).astype(np.float32)
+    (n,) = struct.unpack("<i", take(4))
+    names = []
+    for _ in range(n):
+        (ln,) = struct.unpack("<i", take(4))
+        names.append(take(ln).decode())
+    normals = verts[:, 3:6].copy() if has_n else None
+    tc0 = 6 if has_n else 3
+    texcoords = verts[:, tc0:tc0 + 2].copy() if has_uv else None
+    return Mesh(idx, np.ascontiguousarray(verts[:, :3]), mat, names, normals, texcoords)
+
+
+def save_mesh(path: str, m: Mesh) -> None:
+    with open(path, "wb") as f:
+        f.write(mesh_to_bytes(m))
+
+
+def load_mesh(path: str) -> Mesh:
+    with open(path, "rb") as f:
+        return mesh_from_bytes(f.read())
+
+
+VIEW_FMT = "<3f5f"   # pos xyz, angleX, angleY, fovY, radius, focus
+VIEWSET_SIZE = 10
+
+
+def views_to_bytes(views: list[dict]) -> bytes:
+    vs = list(views) + [views[0]] * (VIEWSET_SIZE - len(views))
+    return b"".join(struct.pack(VIEW_FMT, *v["pos"], v["angle_x"], v["angle_y"], v["fov_y"],
+                                v.get("radius", 0.0), v.get("focus", 1.0)) for v in vs[:VIEWSET_SIZE])
+
+
+def views_from_bytes(data: bytes) -> list[dict]:
+    if len(data) != struct.calcsize(VIEW_FMT) * VIEWSET_SIZE:
+        raise RuntimeError("view set must be 320 bytes")
+    out = []
+    for k in range(VIEWSET_SIZE):
+        f = struct.unpack_from(VIEW_FMT, data, 32 * k)
+        out.append({"pos": list(f[0:3]), "angle_x": f[3], "angle_y": f[4], "fov_y": f[5],
+                    "radius": f[6], "focus": f[7]})
+    return out
+
+
+def ppm_bytes(bgr8: np.ndarray) -> bytes:
+    """P6 image of packed b | g<<8 | r<<16 pixels, bytes written in memory order."""
+    h, w = bgr8.shape
+    px = np.ascontiguousarray(bgr8, dtype="<i4").view(np.uint8).reshape(h, w, 4)[:, :, :3]
+    return b"P6\n%d %d\n255\n" % (w, h) + px.tobytes()
+
+
+# --------------------------------------------------------------- geometry
+def heightfield(cells: int, rng: np.random.Generator, hmax: float, lo=-1.0, hi=1.0) -> tuple:
+    n = cells + 1
+    xs = np.linspace(lo, hi, n, dtype=np.float64)
+    gx, gz = np.meshgrid(xs, xs, indexing="xy")
+    gy = rng.uniform(0.0, hmax, size=(n, n))
+    pos = np.stack([gx, gy, gz], axis=-1).reshape(-1, 3).astype(np.float32)
+    i = np.arange(cells)
+    a = (i[None, :] + n * i[:, None]).ravel()
+    b, c, d = a + 1, a + n, a + n + 1
+    idx = np.concatenate([np.stack([a, c, b], 1), np.stack([b, c, d], 1)], axis=0).astype(np.int32)
+    return idx, pos
+
+
+def quad(p0, p1, p2, p3) -> tuple:
+    pos = np.array([p0, p1, p2, p3], dtype=np.float32)
+    idx = np.array([[0, 1, 2], [0, 2, 3]], dtype=np.int32)
+    return idx, pos
+
+
+def icosphere(subdiv: int) -> tuple:
+    """Unit icosphere: 20 * 4^subdiv triangles, 10 * 4^subdiv + 2 vertices."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+             (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    v = [np.array(p, dtype=np.float64) / np.linalg.norm(p) for p in verts]
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(i, j):
+            key = (min(i, j), max(i, j))
+            if key not in cache:
+                m = v[i] + v[j]
+                v.append(m / np.linalg.norm(m))
+                cache[key] = len(v) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    return np.array(faces, dtype=np.int32), np.array(v, dtype=np.float64)
+
+
+def box_tris(lo, hi) -> tuple:
+    """Closed axis-aligned box: 8 vertices, 12 triangles."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    pos = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                    [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]], dtype=np.float32)
+    idx = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7], [0, 1, 5], [0, 5, 4],
+                    [3, 7, 6], [3, 6, 2], [0, 4, 7], [0, 7, 3], [1, 2, 6], [1, 6, 5]], dtype=np.int32)
+    return idx, pos
+
+
+def merge(parts: list[tuple]) -> Mesh:
+    idxs, poss, off = [], [], 0
+    for idx, pos in parts:
+        idxs.append(np.asarray(idx, dtype=np.int64) + off)
+        poss.append(np.asarray(pos, dtype=np.float32))
+        off += len(pos)
+    return Mesh(np.concatenate(idxs).astype(np.int32), np.concatenate(poss).astype(np.float32))
+
+
+# -------------------------------------------------------------- workloads
+def scene_heightfield_box(seed: int = 1001, cells: int = 64) -> Mesh:
+    """64x64-cell heightfield (heights U[0, 0.2]) on [-1,1]^2 inside the open
+    5-quad box [-1,1] x [0,2] x [-1,1] (floor and four walls, no lid)."""
+    rng = np.random.default_rng(seed)
+    hf = heightfield(cells, rng, 0.2)
+    parts = [hf,
+             quad((-1, 0, -1), (1, 0, -1), (1, 0, 1), (-1, 0, 1)),
+             quad((-1, 0, -1), (-1, 2, -1), (1, 2, -1), (1, 0, -1)),
+             quad((-1, 0, 1), (1, 0, 1), (1, 2, 1), (-1, 2, 1)),
+             quad((-1, 0, -1), (-1, 0, 1), (-1, 2, 1), (-1, 2, -1)),
+             quad((1, 0, -1), (1, 2, -1), (1, 2, 1), (1, 0, 1))]
+    return merge(parts)
+
+
+def scene_spheres(seed: int = 1002, count: int = 256, subdiv: int = 4) -> Mesh:
+    """`count` icospheres (centres U[0,1]^3, radii log-uniform [0.01, 0.1]) over
+    a two-triangle ground quad at y = 0 spanning [-1, 2]^2."""
+    rng = np.random.default_rng(seed)
+    f, v = icosphere(subdiv)
+    centres = rng.uniform(0.0, 1.0, size=(count, 3))
+    radii = np.exp(rng.uniform(math.log(0.01), math.log(0.1), size=count))
+    parts = [(f, (v * r + c).astype(np.float32)) for c, r in zip(centres, radii)]
+    parts.append(quad((-1, 0, -1), (-1, 0, 2), (2, 0, 2), (2, 0, -1)))
+    return merge(parts)
+
+
+def scene_convex_sphere(subdiv: int = 3) -> Mesh:
+    f, v = icosphere(subdiv)
+    return Mesh(f, v.astype(np.float32))
+
+
+def scene_closed_box() -> Mesh:
+    idx, pos = box_tris((-1, -1, -1), (1, 1, 1))
+    return Mesh(idx, pos)
+
+
+# Workload table: scene builder, view (ProtoRay .view semantics), image size,
+# spp, max depth and albedo.
+CONFIGS = {
+    "heightfield_box_512": {
+        "scene": "heightfield_box", "seed": 1001,
+        "view": {"pos": [0.0, 4.5, 1.2], "angle_x": 1.3, "angle_y": 0.0, "fov_y": math.radians(45.0)},
+        "width": 512, "height": 512, "spp": 4, "max_depth": 4, "albedo": 0.8,
+    },
+    "spheres_1080p": {
+        "scene": "spheres", "seed": 1002,
+        "view": {"pos": [0.5, 0.9, 2.6], "angle_x": 0.25, "angle_y": 0.0, "fov_y": math.radians(45.0)},
+        "width": 1920, "height": 1080, "spp": 16, "max_depth": 8, "albedo": 0.7,
+    },
+}
+
+
+def build_scene(name: str) -> Mesh:
+    cfg = CONFIGS[name]
+    if cfg["scene"] == "heightfield_box":
+        return scene_heightfield_box(cfg["seed"])
+    if cfg["scene"] == "spheres":
+        return scene_spheres(cfg["seed"])
+    raise KeyError(name)

@@ -1,0 +1,177 @@
+/*
+ * prt_hip.h -- C ABI of libprt_hip.so, the MI355X (gfx950) wavefront diffuse
+ * path tracer that sits behind ProtoRay's Device API as "-dev hip".
+ *
+ * Conventions
+ *   - Every function returns int status: 0 = ok, nonzero = error.  The text of
+ *     the last error of a context is prt_hip_last_error(ctx).  No C++
+ *     exception crosses this boundary.
+ *   - The caller owns host arrays; the context owns every device allocation it
+ *     makes and frees them in prt_hip_destroy.  One HIP stream per context.
+ *     A context is not thread-safe; several contexts may coexist.
+ *   - Pointers named d_* are device pointers (HBM), everything else is host.
+ *
+ * Reference interfaces replaced (paths are in embree-benchmark-protoray):
+ *   render/device.h:29-54                     class Device (plugin point)
+ *   render/device_cuda.cpp:121-168            DeviceCuda::initScene (mesh upload)
+ *   accel/optix/optix_intersector_stream_cuda.cpp:23-75   BVH build (+ buildMs/buildMprim)
+ *   core/intersector_stream_cuda.h:27-34      IntersectorStreamCuda::intersect
+ *   render/device_cuda.cpp:175-200            DeviceCuda::initCamera
+ *   image/frame_buffer_cuda.cpp:22-50         FrameBufferCuda (init/clear/map)
+ *   render/diffuse_renderer_cuda.cu:159-212   DiffuseRendererCuda::render
+ *   render/primary_renderer_cuda.cu:124-153   PrimaryRendererCuda::render
+ *   image/accum_buffer_cuda.cu:22-44          accumBufferUpdateKernel (tonemap)
+ */
+#ifndef PRT_HIP_H
+#define PRT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PRT_HIP_ABI_VERSION 1
+
+typedef struct prt_hip_ctx prt_hip_ctx;
+
+/* Ray record, 32 B.  Same field order as RayCuda (core/ray_cuda.cuh:46-52).
+ * A hit is accepted for tnear <= t <= tfar.  tfar < 0 marks a dead ray that
+ * is not traced (its hit reports a miss). */
+typedef struct prt_ray {
+    float org[3];
+    float tnear;
+    float dir[3];
+    float tfar;
+} prt_ray;
+
+/* Hit record, 16 B.  Same layout and miss encoding as HitCuda
+ * (core/ray_cuda.cuh:23-34): miss <=> t < 0 (t = -1, prim_id = -1).
+ * u, v are barycentrics with p = (1-u-v) p0 + u p1 + v p2
+ * (geometry/triangle_mesh.cpp:69-71). */
+typedef struct prt_hit {
+    float t;
+    int32_t prim_id;
+    float u;
+    float v;
+} prt_hit;
+
+/* Camera, the data of ThinLensCameraCuda (camera/thin_lens_camera_cuda.h:23-30).
+ * Only the pinhole form (lens_radius == 0) is supported. */
+typedef struct prt_camera {
+    float origin[3];
+    float image_o[3];
+    float image_dx[3];
+    float image_dy[3];
+    float basis_u[3];
+    float basis_v[3];
+    float lens_radius;
+    float focal_distance;
+} prt_camera;
+
+/* Stats of prt_hip_bvh_build (the buildMs / buildMprim keys of
+ * accel/optix/optix_intersector_stream_cuda.cpp:61-70). */
+typedef struct prt_build_stats {
+    double build_ms;      /* device time of one build                      */
+    double build_mprim;   /* triangles / build time, millions per second   */
+    int64_t node_count;   /* BVH8 nodes                                    */
+    int64_t tri_count;    /* triangle records (== triangles)               */
+    int32_t depth;        /* BVH8 levels                                   */
+    int32_t reserved;
+} prt_build_stats;
+
+/* Stats of one prt_hip_render call: the mray / ray keys of
+ * render/device_cuda.cpp:99-107.  rays counts every ray handed to the
+ * closest-hit query, primary and secondary (render/diffuse_renderer_cuda.cu:186). */
+typedef struct prt_render_stats {
+    int64_t rays;
+    double ms;
+    double mray;
+    int64_t paths;        /* live primary rays = W*H                       */
+    int64_t queue[16];    /* rays traced at depth k (k <= max_depth)       */
+} prt_render_stats;
+
+/* Per-ray traversal counters (the PROFILE_MODE nodes/prims of
+ * render/renderer_single.h:93-96), filled by the instrumented replay. */
+typedef struct prt_trace_counters {
+    int64_t rays;
+    int64_t nodes;        /* BVH8 nodes visited                             */
+    int64_t tris;         /* triangle tests                                 */
+    int64_t spills;       /* short-stack entries moved to the global spill  */
+} prt_trace_counters;
+
+enum {
+    PRT_RENDERER_PRIMARY = 0,   /* "primaryHip": (N+1)/2 on hit, 0.05 on miss */
+    PRT_RENDERER_DIFFUSE = 1    /* "diffuseHip": diffuse path tracing         */
+};
+
+/* --- context ------------------------------------------------------------ */
+int         prt_hip_abi_version(void);
+int         prt_hip_create(int device_index, prt_hip_ctx** out_ctx);
+int         prt_hip_destroy(prt_hip_ctx* ctx);
+const char* prt_hip_last_error(prt_hip_ctx* ctx);
+/* Device name (DeviceCuda::getInfo, render/device_cuda.cpp:40-48). */
+int         prt_hip_device_name(prt_hip_ctx* ctx, char* buf, int buf_len);
+
+/* --- scene (DeviceCuda::initScene) -------------------------------------- */
+/* idx3: 3*T int32 vertex indices; pos3: 3*V float positions (float3 repack of
+ * the strided .mesh vertex attributes, render/device_cuda.cpp:135-142). */
+int prt_hip_scene_upload(prt_hip_ctx* ctx, const int32_t* idx3, int64_t tri_count,
+                         const float* pos3, int64_t vertex_count);
+
+/* --- acceleration structure (OptixIntersectorStreamCuda ctor) ------------ */
+/* LBVH on the GPU: Morton codes, LSD radix sort, Karras radix tree, bottom-up
+ * boxes, collapse to a BVH8 with 8-bit quantised child boxes. */
+int prt_hip_bvh_build(prt_hip_ctx* ctx, prt_build_stats* out_stats);
+
+/* --- closest hit (IntersectorStreamCuda::intersect) ---------------------- */
+/* Host arrays; copies in, traces, copies out (parity entry). */
+int prt_hip_intersect_closest(prt_hip_ctx* ctx, const prt_ray* rays, prt_hit* hits, int64_t n);
+/* Device arrays, on the context's stream; asynchronous. */
+int prt_hip_intersect_closest_device(prt_hip_ctx* ctx, const prt_ray* d_rays, prt_hit* d_hits,
+                                     int64_t n);
+/* Same query on the instrumented kernel; adds per-ray node/triangle counts. */
+int prt_hip_intersect_counters(prt_hip_ctx* ctx, const prt_ray* rays, prt_hit* hits, int64_t n,
+                               prt_trace_counters* out_counters);
+
+/* --- camera / frame (DeviceCuda::initCamera / initFrame / clearFrame) ---- */
+/* Pinhole camera of a view: RenderWindow::makeCamera
+ * (main/render/render_window.cpp:307-325: basis = Basis3(one).rotateV(angleY)
+ * .rotateU(angleX)) followed by the PinholeCamera constructor
+ * (camera/pinhole_camera.h:34-48).  Host-only; needs no context. */
+int prt_hip_camera_from_view(const float position[3], float angle_x, float angle_y, float fov_y,
+                             float aspect_ratio, prt_camera* out_camera);
+int prt_hip_camera_set(prt_hip_ctx* ctx, const prt_camera* camera);
+int prt_hip_frame_init(prt_hip_ctx* ctx, int width, int height);
+int prt_hip_frame_clear(prt_hip_ctx* ctx);    /* also resets the pass counter */
+
+/* --- rendering (DiffuseRendererCuda::render / PrimaryRendererCuda::render) */
+/* One sample per pixel per call, pass = number of calls since the last clear
+ * (render/diffuse_renderer_cuda.cu:210).  albedo is the grey reflectance R
+ * (the reference hard-codes 0.8, render/diffuse_integrator_single.h:43). */
+int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo,
+                   prt_render_stats* out_stats);
+/* Sets the pass index the next prt_hip_render call uses. */
+int prt_hip_set_pass(prt_hip_ctx* ctx, int pass);
+
+/* --- frame buffer read-back -------------------------------------------- */
+/* Raw accumulation buffer, float4 (r, g, b, weight) per pixel, row-major. */
+int prt_hip_accum_read(prt_hip_ctx* ctx, float* dst_rgba);
+/* Tonemapped 32-bit pixels b | g<<8 | r<<16 (accumBufferUpdateKernel). */
+int prt_hip_frame_read_bgr8(prt_hip_ctx* ctx, int32_t* dst);
+
+/* --- inspection (tests and the roofline model) -------------------------- */
+/* Primary rays of a pass exactly as the diffuse renderer generates them,
+ * one per pixel in row-major pixel order, plus their sampler states. */
+int prt_hip_debug_raygen(prt_hip_ctx* ctx, int pass, prt_ray* rays, uint32_t* states);
+/* Sizes of the built BVH8 (node and triangle-record arrays). */
+int prt_hip_bvh_info(prt_hip_ctx* ctx, int64_t* node_count, int64_t* tri_count, int32_t* depth);
+/* Copies the BVH8: nodes (80 B each), triangle records (9 floats each) and
+ * the original triangle id of each record. */
+int prt_hip_bvh_read(prt_hip_ctx* ctx, void* nodes, float* tri_records, int32_t* prim_ids);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PRT_HIP_H */

@@ -1,0 +1,954 @@
+// oracle.cpp -- CPU restatement of ProtoRay's diffuse path, used ONLY as the
+// checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+// It is test infrastructure: nothing in the shipped library calls it.
+//
+// What it restates (reference paths in embree-benchmark-protoray):
+//   sampler          sampling/random_sampler.h:44-73, math/hash.h:23-72, math/math.h:179
+//   camera           camera/pinhole_camera.h:34-55, main/render/render_window.cpp:307-325,
+//                    math/basis3.h:53-79, math/vec3.h:409-412
+//   primary rays     render/renderer_single.h:102-123
+//   diffuse path     render/diffuse_integrator_single.h:41-69, core/ray.h:47-52,71-76,
+//                    geometry/triangle_mesh.cpp:248-273, math/basis3.h:128-145,
+//                    sampling/shape_sampler.h:50-74
+//   primary shading  render/primary_integrator_single.h:34-47
+//   accumulation     image/accum_buffer.h:124-147
+//   stream renderer  render/diffuse_renderer_stream.h:82-225, render/renderer_stream.h:69-88
+//   tasking          sys/tasking.h:33-52 (atomic tile counter over threads)
+//
+// Intersection: the reference delegates it to Embree 3 / OptiX Prime, which are
+// absent here.  Two restatements:
+//   - FP64: watertight test (Woop, Benthin, Wald, JCGT 2013) in double over a
+//     BVH2, closest (t, primId) pair; the north-star parity reference.
+//   - kernel numerics: the same test in fp32 with the double fallback for zero
+//     edge functions and a (t, primId) lexicographic minimum, evaluated over
+//     every triangle the double-precision box test admits; the GPU kernel must
+//     reproduce it bit for bit.
+// Numerics: compile with -ffp-contract=off (no fused operations unless spelled
+// fmaf), IEEE scalar SSE arithmetic.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+// ----------------------------------------------------------------- sampler
+uint32_t rotl(uint32_t x, uint32_t r) { return (x << r) | (x >> (32u - r)); }
+
+uint32_t mmix(uint32_t h, uint32_t k)
+{
+    k *= 0xcc9e2d51u;
+    k = rotl(k, 15u);
+    k *= 0x1b873593u;
+    h ^= k;
+    return rotl(h, 13u) * 5u + 0xe6546b64u;
+}
+
+uint32_t mfin(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+uint32_t seed_of(int pass, int pixel) { return mfin(mmix(mmix(0u, (uint32_t)pixel), (uint32_t)pass)); }
+
+struct Sampler {
+    uint32_t s;
+    float next()
+    {
+        s = s * 1664525u + 1013904223u;
+        return (float)s * 0x1.0p-32f;
+    }
+};
+
+// ----------------------------------------------------------------- vectors
+struct V3 {
+    float x, y, z;
+};
+V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+V3 cross(V3 a, V3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+V3 normalized(V3 a)
+{
+    float inv = 1.0f / std::sqrt(dot(a, a));
+    return a * inv;
+}
+float at(V3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+struct D3 {
+    double x, y, z;
+};
+D3 d3(V3 a) { return D3{a.x, a.y, a.z}; }
+D3 operator-(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+D3 operator+(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+D3 operator*(D3 a, double s) { return D3{a.x * s, a.y * s, a.z * s}; }
+double dotd(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+D3 crossd(D3 a, D3 b) { return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+double atd(D3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+// ----------------------------------------------------------------- sin/cos
+// sin(2 pi v), cos(2 pi v): quadrant k = (int)(4v + 0.5), x = (4v - k) pi/2,
+// Taylor polynomials (x^9 for sin, x^10 for cos) by Horner with fmaf.
+void sincos_2pi_kernel(float v, float& so, float& co)
+{
+    float w = v * 4.0f;
+    int k = (int)(w + 0.5f);
+    float f = w - (float)k;
+    float x = f * 1.57079632679489662f;
+    float x2 = x * x;
+    float ps = 2.75573192239858907e-6f;
+    ps = std::fma(ps, x2, -1.98412698412698413e-4f);
+    ps = std::fma(ps, x2, 8.33333333333333333e-3f);
+    ps = std::fma(ps, x2, -1.66666666666666667e-1f);
+    float sx = std::fma(ps * x2, x, x);
+    float pc = -2.75573192239858907e-7f;
+    pc = std::fma(pc, x2, 2.48015873015873016e-5f);
+    pc = std::fma(pc, x2, -1.38888888888888889e-3f);
+    pc = std::fma(pc, x2, 4.16666666666666667e-2f);
+    pc = std::fma(pc, x2, -0.5f);
+    float cx = std::fma(pc, x2, 1.0f);
+    switch (k & 3) {
+    case 0: so = sx; co = cx; break;
+    case 1: so = cx; co = -sx; break;
+    case 2: so = -sx; co = -cx; break;
+    default: so = -cx; co = sx; break;
+    }
+}
+
+// ----------------------------------------------------------------- scene
+struct Tri {
+    V3 p0, p1, p2;
+};
+
+struct Node {      // BVH2 node over float boxes
+    float lo[3], hi[3];
+    int left;      // internal: index of left child (right = left + 1); leaf: -1
+    int first, count;
+};
+
+struct Scene {
+    std::vector<Tri> tris;
+    std::vector<int> order;   // leaf order -> triangle id
+    std::vector<Node> nodes;
+};
+
+void tri_bounds(const Tri& t, float lo[3], float hi[3])
+{
+    for (int k = 0; k < 3; ++k) {
+        float a = at(t.p0, k), b = at(t.p1, k), c = at(t.p2, k);
+        lo[k] = std::min(std::min(a, b), c);
+        hi[k] = std::max(std::max(a, b), c);
+    }
+}
+
+// binned SAH build, 16 bins, leaves of up to 4 triangles
+void build_bvh(Scene& sc)
+{
+    const int n = (int)sc.tris.size();
+    sc.order.resize(n);
+    for (int i = 0; i < n; ++i) sc.order[i] = i;
+    std::vector<float> blo(3 * (size_t)n), bhi(3 * (size_t)n), cen(3 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        tri_bounds(sc.tris[i], &blo[3 * i], &bhi[3 * i]);
+        for (int k = 0; k < 3; ++k) cen[3 * i + k] = 0.5f * (blo[3 * i + k] + bhi[3 * i + k]);
+    }
+    sc.nodes.clear();
+    sc.nodes.reserve(2 * (size_t)std::max(n, 1));
+    struct Task { int node, first, count; };
+    std::vector<Task> stack;
+    sc.nodes.push_back(Node{});
+    stack.push_back({0, 0, n});
+    while (!stack.empty()) {
+        Task t = stack.back();
+        stack.pop_back();
+        Node nd{};
+        for (int k = 0; k < 3; ++k) { nd.lo[k] = INFINITY; nd.hi[k] = -INFINITY; }
+        float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int i = t.first; i < t.first + t.count; ++i) {
+            int id = sc.order[i];
+            for (int k = 0; k < 3; ++k) {
+                nd.lo[k] = std::min(nd.lo[k], blo[3 * id + k]);
+                nd.hi[k] = std::max(nd.hi[k], bhi[3 * id + k]);
+                clo[k] = std::min(clo[k], cen[3 * id + k]);
+                chi[k] = std::max(chi[k], cen[3 * id + k]);
+            }
+        }
+        nd.left = -1;
+        nd.first = t.first;
+        nd.count = t.count;
+        int split = -1;
+        if (t.count > 4) {
+            // pick the axis/bin with the lowest SAH cost
+            const int B = 16;
+            double best = INFINITY;
+            int best_axis = -1, best_bin = -1;
+            for (int k = 0; k < 3; ++k) {
+                float ext = chi[k] - clo[k];
+                if (!(ext > 0.0f)) continue;
+                int cnt[B] = {0};
+                float bl[B][3], bh[B][3];
+                for (int b = 0; b < B; ++b)
+                    for (int a = 0; a < 3; ++a) { bl[b][a] = INFINITY; bh[b][a] = -INFINITY; }
+                for (int i = t.first; i < t.first + t.count; ++i) {
+                    int id = sc.order[i];
+                    int b = std::min(B - 1, (int)((cen[3 * id + k] - clo[k]) / ext * B));
+                    cnt[b]++;
+                    for (int a = 0; a < 3; ++a) {
+                        bl[b][a] = std::min(bl[b][a], blo[3 * id + a]);
+                        bh[b][a] = std::max(bh[b][a], bhi[3 * id + a]);
+                    }
+                }
+                auto area = [](const float l[3], const float h[3]) {
+                    double dx = h[0] - l[0], dy = h[1] - l[1], dz = h[2] - l[2];
+                    return dx * dy + dy * dz + dz * dx;
+                };
+                double right_area[B];
+                int right_cnt[B];
+                float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
+                int rc = 0;
+                for (int b = B - 1; b > 0; --b) {
+                    rc += cnt[b];
+                    for (int a = 0; a < 3; ++a) { rl[a] = std::min(rl[a], bl[b][a]); rh[a] = std::max(rh[a], bh[b][a]); }
+                    right_area[b] = rc ? area(rl, rh) : 0.0;
+                    right_cnt[b] = rc;
+                }
+                float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
+                int lc = 0;
+                for (int b = 0; b < B - 1; ++b) {
+                    lc += cnt[b];
+                    for (int a = 0; a < 3; ++a) { ll[a] = std::min(ll[a], bl[b][a]); lh[a] = std::max(lh[a], bh[b][a]); }
+                    if (lc == 0 || right_cnt[b + 1] == 0) continue;
+                    double c = (lc ? area(ll, lh) * lc : 0.0) + right_area[b + 1] * right_cnt[b + 1];
+                    if (c < best) { best = c; best_axis = k; best_bin = b; }
+                }
+            }
+            int mid;
+            if (best_axis >= 0) {
+                int k = best_axis;
+                float ext = chi[k] - clo[k];
+                auto it = std::partition(sc.order.begin() + t.first, sc.order.begin() + t.first + t.count,
+                                         [&](int id) {
+                                             int b = std::min(B - 1, (int)((cen[3 * id + k] - clo[k]) / ext * B));
+                                             return b <= best_bin;
+                                         });
+                mid = (int)(it - sc.order.begin());
+            } else {
+                mid = t.first + t.count / 2;  // all centroids equal: split by count
+            }
+            if (mid == t.first || mid == t.first + t.count) mid = t.first + t.count / 2;
+            split = mid;
+        }
+        if (split >= 0) {
+            nd.left = (int)sc.nodes.size();
+            sc.nodes.push_back(Node{});
+            sc.nodes.push_back(Node{});
+            stack.push_back({nd.left, t.first, split - t.first});
+            stack.push_back({nd.left + 1, split, t.first + t.count - split});
+        }
+        sc.nodes[t.node] = nd;
+    }
+}
+
+// Slab test in double against the box padded by 1e-5 (1 + max |coordinate|
+// of box and origin).  The padding exceeds the lateral error of the fp32
+// watertight test (about 2^-23 times the distance), so every triangle that
+// test could accept is visited: the walk is equivalent to testing all
+// triangles, for both the fp32 and the FP64 queries.
+bool box_hit_d(const Node& nd, D3 o, D3 inv, double tmin, double tmax, double& tnear)
+{
+    double m = std::max(std::max(std::fabs(o.x), std::fabs(o.y)), std::fabs(o.z));
+    for (int k = 0; k < 3; ++k) m = std::max(m, std::max(std::fabs((double)nd.lo[k]), std::fabs((double)nd.hi[k])));
+    const double pad = 1e-5 * (1.0 + m);
+    double t0 = tmin, t1 = tmax;
+    for (int k = 0; k < 3; ++k) {
+        double a = ((nd.lo[k] - pad) - atd(o, k)) * atd(inv, k);
+        double b = ((nd.hi[k] + pad) - atd(o, k)) * atd(inv, k);
+        if (a > b) std::swap(a, b);
+        double sl = 1e-12 * (std::fabs(a) + std::fabs(b)) + 1e-300;
+        t0 = std::max(t0, a - sl);
+        t1 = std::min(t1, b + sl);
+    }
+    tnear = t0;
+    return t0 <= t1;
+}
+
+D3 safe_inv(D3 d)
+{
+    auto f = [](double x) { return std::fabs(x) < 1e-300 ? (x < 0 ? -1e300 : 1e300) : 1.0 / x; };
+    return D3{f(d.x), f(d.y), f(d.z)};
+}
+
+// Visit every leaf whose box the ray can reach within [tmin, *tmax];
+// *tmax may shrink during the walk (closest-first order is not required).
+template <class F>
+void walk(const Scene& sc, D3 o, D3 d, double tmin, const double* tmax, F&& leaf)
+{
+    if (sc.nodes.empty() || sc.tris.empty()) return;
+    D3 inv = safe_inv(d);
+    int stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const Node& nd = sc.nodes[stack[--sp]];
+        double tn;
+        if (!box_hit_d(nd, o, inv, tmin, *tmax, tn)) continue;
+        if (nd.left < 0) {
+            for (int i = nd.first; i < nd.first + nd.count; ++i) leaf(sc.order[i]);
+        } else {
+            stack[sp++] = nd.left + 1;
+            stack[sp++] = nd.left;
+        }
+    }
+}
+
+// ----------------------------------------------------------- FP64 triangle
+struct Hit64 {
+    bool hit;
+    double t, u, v;
+};
+
+Hit64 tri_test_d(D3 o, D3 d, const Tri& tr, double tmin, double tmax)
+{
+    Hit64 h{false, 0, 0, 0};
+    double ax_ = std::fabs(d.x), ay_ = std::fabs(d.y), az_ = std::fabs(d.z);
+    int kz = 0;
+    double m = ax_;
+    if (ay_ > m) { kz = 1; m = ay_; }
+    if (az_ > m) kz = 2;
+    int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+    if (atd(d, kz) < 0.0) std::swap(kx, ky);
+    double sx = atd(d, kx) / atd(d, kz), sy = atd(d, ky) / atd(d, kz), sz = 1.0 / atd(d, kz);
+    D3 a = d3(tr.p0) - o, b = d3(tr.p1) - o, c = d3(tr.p2) - o;
+    double Ax = atd(a, kx) - sx * atd(a, kz), Ay = atd(a, ky) - sy * atd(a, kz);
+    double Bx = atd(b, kx) - sx * atd(b, kz), By = atd(b, ky) - sy * atd(b, kz);
+    double Cx = atd(c, kx) - sx * atd(c, kz), Cy = atd(c, ky) - sy * atd(c, kz);
+    double U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if ((U < 0 || V < 0 || W < 0) && (U > 0 || V > 0 || W > 0)) return h;
+    double det = U + V + W;
+    if (det == 0.0) return h;
+    double T = sz * (U * atd(a, kz) + V * atd(b, kz) + W * atd(c, kz));
+    double t = T / det;
+    if (!(t >= tmin && t <= tmax)) return h;
+    h.hit = true;
+    h.t = t;
+    h.u = V / det;
+    h.v = W / det;
+    return h;
+}
+
+// distance from the hit point (barycentrics u, v) to the nearest edge
+double edge_distance(const Tri& tr, double u, double v)
+{
+    D3 p0 = d3(tr.p0), p1 = d3(tr.p1), p2 = d3(tr.p2);
+    D3 n = crossd(p1 - p0, p2 - p0);
+    double a2 = std::sqrt(dotd(n, n));  // twice the area
+    double w = 1.0 - u - v;
+    auto len = [](D3 x) { return std::sqrt(dotd(x, x)); };
+    double h0 = a2 / std::max(len(p2 - p1), 1e-300);  // opposite p0, weight w
+    double h1 = a2 / std::max(len(p0 - p2), 1e-300);  // opposite p1, weight u
+    double h2 = a2 / std::max(len(p1 - p0), 1e-300);  // opposite p2, weight v
+    return std::min(std::min(std::fabs(w) * h0, std::fabs(u) * h1), std::fabs(v) * h2);
+}
+
+// shortest distance between the ray's line and the triangle's edges
+double ray_edge_distance(D3 o, D3 d, const Tri& tr)
+{
+    D3 p[3] = {d3(tr.p0), d3(tr.p1), d3(tr.p2)};
+    double best = INFINITY;
+    for (int e = 0; e < 3; ++e) {
+        D3 a = p[e], b = p[(e + 1) % 3];
+        D3 u = b - a;
+        D3 w0 = o - a;
+        double A = dotd(d, d), B = dotd(d, u), C = dotd(u, u), D = dotd(d, w0), E = dotd(u, w0);
+        double den = A * C - B * B;
+        double sc_, tc;
+        if (den < 1e-300) { sc_ = 0.0; tc = C > 0 ? E / C : 0.0; }
+        else { sc_ = (B * E - C * D) / den; tc = (A * E - B * D) / den; }
+        tc = std::min(1.0, std::max(0.0, tc));
+        sc_ = std::max(0.0, (B * tc - D) / std::max(A, 1e-300));
+        D3 q = (o + d * sc_) - (a + u * tc);
+        best = std::min(best, std::sqrt(dotd(q, q)));
+    }
+    return best;
+}
+
+struct Closest64 {
+    int prim = -1;
+    double t = INFINITY, u = 0, v = 0;
+    int prim2 = -1;        // runner-up distinct triangle
+    double t2 = INFINITY;
+};
+
+Closest64 closest_fp64(const Scene& sc, V3 of, V3 df, float tminf, float tmaxf, bool want_second)
+{
+    D3 o = d3(of), d = d3(df);
+    Closest64 r;
+    double bound = tmaxf;
+    walk(sc, o, d, tminf, &bound, [&](int id) {
+        Hit64 h = tri_test_d(o, d, sc.tris[id], tminf, tmaxf);
+        if (!h.hit) return;
+        bool better = h.t < r.t || (h.t == r.t && id < r.prim);
+        if (better) {
+            if (r.prim >= 0) { r.prim2 = r.prim; r.t2 = r.t; }
+            r.prim = id; r.t = h.t; r.u = h.u; r.v = h.v;
+        } else if (h.t < r.t2 || (h.t == r.t2 && id < r.prim2)) {
+            r.prim2 = id; r.t2 = h.t;
+        }
+        bound = want_second ? std::min((double)tmaxf, r.t2) : std::min((double)tmaxf, r.t);
+    });
+    return r;
+}
+
+// ----------------------------------------------------- kernel-numerics fp32
+struct WRay {
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+
+WRay wsetup(V3 d)
+{
+    WRay w;
+    float ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+    int kz = 0;
+    float m = ax;
+    if (ay > m) { kz = 1; m = ay; }
+    if (az > m) kz = 2;
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    float dz = at(d, kz);
+    if (dz < 0.0f) std::swap(kx, ky);
+    w.kx = kx; w.ky = ky; w.kz = kz;
+    w.sx = at(d, kx) / dz;
+    w.sy = at(d, ky) / dz;
+    w.sz = 1.0f / dz;
+    return w;
+}
+
+bool tri_test_f(const WRay& w, V3 org, const Tri& tr, float tmin, float tbest, float& t_out, float& vn,
+                float& wn, float& det_out)
+{
+    V3 a = tr.p0 - org, b = tr.p1 - org, c = tr.p2 - org;
+    float az_ = at(a, w.kz), bz_ = at(b, w.kz), cz_ = at(c, w.kz);
+    float ax = at(a, w.kx) - w.sx * az_, ay = at(a, w.ky) - w.sy * az_;
+    float bx = at(b, w.kx) - w.sx * bz_, by = at(b, w.ky) - w.sy * bz_;
+    float cx = at(c, w.kx) - w.sx * cz_, cy = at(c, w.ky) - w.sy * cz_;
+    float U = cx * by - cy * bx, V = ax * cy - ay * cx, W = bx * ay - by * ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)cx * (double)by - (double)cy * (double)bx);
+        V = (float)((double)ax * (double)cy - (double)ay * (double)cx);
+        W = (float)((double)bx * (double)ay - (double)by * (double)ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = (U + V) + W;
+    if (det == 0.0f) return false;
+    float azs = w.sz * az_, bzs = w.sz * bz_, czs = w.sz * cz_;
+    float T = (U * azs + V * bzs) + W * czs;
+    float tdet = det < 0.0f ? -T : T;
+    float adet = std::fabs(det);
+    if (tdet < tmin * adet) return false;
+    if (tdet > (tbest * adet) * 1.00000095367431641f) return false;
+    float t = T / det;
+    if (!(t >= tmin)) return false;
+    t_out = t; vn = V; wn = W; det_out = det;
+    return true;
+}
+
+struct Closest32 {
+    int prim = -1;
+    float t = 0, u = 0, v = 0;
+};
+
+Closest32 closest_fp32(const Scene& sc, V3 o, V3 d, float tmin, float tmax)
+{
+    Closest32 r;
+    r.t = tmax;
+    WRay w = wsetup(d);
+    double bound = tmax;
+    walk(sc, d3(o), d3(d), tmin, &bound, [&](int id) {
+        float t, vn, wn, det;
+        if (!tri_test_f(w, o, sc.tris[id], tmin, r.t, t, vn, wn, det)) return;
+        if (t < r.t || r.prim < 0 || (t == r.t && id < r.prim)) {
+            r.t = t; r.prim = id; r.u = vn / det; r.v = wn / det;
+            bound = std::max((double)t * (1.0 + 1e-6), (double)t);
+        }
+    });
+    return r;
+}
+
+// Distance to the winning triangle's plane in double, rounded to float:
+// t = ((p0 - o) . n) / (d . n), n = (p1 - p0) x (p2 - p0).  Differences of
+// floats and their products are exact in double; each sum rounds once.
+// Falls back to the fp32 t if the plane is parallel to the ray.
+float refine_t(const Tri& tr, V3 o, V3 d, float t32)
+{
+    double e1x = (double)tr.p1.x - (double)tr.p0.x, e1y = (double)tr.p1.y - (double)tr.p0.y,
+           e1z = (double)tr.p1.z - (double)tr.p0.z;
+    double e2x = (double)tr.p2.x - (double)tr.p0.x, e2y = (double)tr.p2.y - (double)tr.p0.y,
+           e2z = (double)tr.p2.z - (double)tr.p0.z;
+    double nx = e1y * e2z - e1z * e2y;
+    double ny = e1z * e2x - e1x * e2z;
+    double nz = e1x * e2y - e1y * e2x;
+    double wx = (double)tr.p0.x - (double)o.x, wy = (double)tr.p0.y - (double)o.y,
+           wz = (double)tr.p0.z - (double)o.z;
+    double num = (wx * nx + wy * ny) + wz * nz;
+    double den = ((double)d.x * nx + (double)d.y * ny) + (double)d.z * nz;
+    if (den == 0.0) return t32;
+    double t = num / den;
+    if (!(t >= 0.0)) return t32;
+    return (float)t;
+}
+
+// ----------------------------------------------------------------- camera
+struct Cam {
+    V3 origin, image_o, image_dx, image_dy;
+};
+
+Cam cam_from(const float* c)
+{
+    return Cam{v3(c[0], c[1], c[2]), v3(c[3], c[4], c[5]), v3(c[6], c[7], c[8]), v3(c[9], c[10], c[11])};
+}
+
+V3 cam_dir(const Cam& c, float ix, float iy)
+{
+    V3 d;
+    d.x = (c.image_o.x + ix * c.image_dx.x) + iy * c.image_dy.x;
+    d.y = (c.image_o.y + ix * c.image_dx.y) + iy * c.image_dy.y;
+    d.z = (c.image_o.z + ix * c.image_dx.z) + iy * c.image_dy.z;
+    return normalized(d);
+}
+
+// ----------------------------------------------------------------- shading
+V3 facing_normal(const Tri& tr, V3 dir)
+{
+    V3 ng = normalized(cross(tr.p1 - tr.p0, tr.p2 - tr.p0));
+    if (dot(ng, dir) > 0.0f) ng = v3(-ng.x, -ng.y, -ng.z);
+    return ng;
+}
+
+// numerics 0: kernel fp32 (polynomial sin/cos); 1: reference (libm sinf/cosf)
+void bounce(V3 org, V3 dir, float t, V3 n, float s1, float s2, int numerics, V3& no, V3& nd)
+{
+    V3 p = v3(org.x + t * dir.x, org.y + t * dir.y, org.z + t * dir.z);
+    float eps = std::max(t, std::max(std::max(std::fabs(p.x), std::fabs(p.y)), std::fabs(p.z))) * 0x1.fp-18f;
+    V3 u0 = v3(0.0f, n.z, -n.y), u1 = v3(-n.z, 0.0f, n.x);
+    V3 u = normalized(dot(u0, u0) > dot(u1, u1) ? u0 : u1);
+    V3 v = normalized(cross(n, u));
+    float ct = std::sqrt(s1), st = std::sqrt(1.0f - s1);
+    float sp, cp;
+    if (numerics == 0) {
+        sincos_2pi_kernel(s2, sp, cp);
+    } else {
+        float phi = 2.0f * 3.14159265358979323846f * s2;
+        sp = std::sin(phi);
+        cp = std::cos(phi);
+    }
+    float lx = cp * st, ly = sp * st, lz = ct;
+    nd = v3((lx * u.x + ly * v.x) + lz * n.x, (lx * u.y + ly * v.y) + lz * n.y, (lx * u.z + ly * v.z) + lz * n.z);
+    no = v3(p.x + eps * nd.x, p.y + eps * nd.y, p.z + eps * nd.z);
+}
+
+struct HitAny {
+    int prim;
+    float t;
+};
+
+HitAny closest_any(const Scene& sc, V3 o, V3 d, int numerics)
+{
+    if (numerics == 0) {
+        Closest32 c = closest_fp32(sc, o, d, 0.0f, 3.402823466e38f);
+        return HitAny{c.prim, c.prim >= 0 ? refine_t(sc.tris[c.prim], o, d, c.t) : c.t};
+    }
+    Closest64 c = closest_fp64(sc, o, d, 0.0f, 3.402823466e38f, false);
+    return HitAny{c.prim, (float)c.t};
+}
+
+template <class F>
+void parallel_tiles(int ntiles, int threads, F&& f)
+{
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+        for (;;) {
+            int t = next.fetch_add(1);
+            if (t >= ntiles) break;
+            f(t);
+        }
+    };
+    if (threads <= 1) { worker(); return; }
+    std::vector<std::thread> pool;
+    for (int i = 0; i < threads; ++i) pool.emplace_back(worker);
+    for (auto& th : pool) th.join();
+}
+
+int default_threads(int threads)
+{
+    if (threads > 0) return threads;
+    unsigned h = std::thread::hardware_concurrency();
+    return h ? (int)h : 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t orc_sampler_seed(int pass, int pixel) { return seed_of(pass, pixel); }
+
+// n draws after the seed: states[i] and the unorm value of each
+void orc_sampler_sequence(int pass, int pixel, int n, uint32_t* states, float* values)
+{
+    Sampler s{seed_of(pass, pixel)};
+    for (int i = 0; i < n; ++i) {
+        float v = s.next();
+        if (states) states[i] = s.s;
+        if (values) values[i] = v;
+    }
+}
+
+void orc_sincos_2pi(const float* v, int64_t n, float* s, float* c)
+{
+    for (int64_t i = 0; i < n; ++i) sincos_2pi_kernel(v[i], s[i], c[i]);
+}
+
+// Pinhole camera of a view (render_window.cpp:307-325 + pinhole_camera.h:34-48):
+// basis = Basis3(one).rotateV(angleY).rotateU(angleX); out = origin, imageO,
+// imageDx, imageDy, basisU, basisV (18 floats).
+void orc_camera_from_view(const float* pos, float angle_x, float angle_y, float fov_y, float aspect, float* out)
+{
+    auto rot = [](V3 vec, V3 axis, float ang) {
+        float c = std::cos(ang), s = std::sin(ang);
+        return (vec * c + (axis * dot(axis, vec)) * (1.0f - c)) + cross(vec, axis) * s;
+    };
+    V3 V = v3(0, 1, 0), N = v3(0, 0, 1);
+    // rotateV(angleY): N' = rotate(N, V, a), U' = cross(V, N')
+    V3 N1 = rot(N, V, angle_y);
+    V3 U1 = cross(V, N1);
+    V3 V1 = V;
+    // rotateU(angleX): V'' = rotate(V1, U1, a), N'' = cross(U1, V'')
+    V3 V2 = rot(V1, U1, angle_x);
+    V3 N2 = cross(U1, V2);
+    V3 U2 = U1;
+    float hh = std::tan(0.5f * fov_y);
+    float hw = hh * aspect;
+    auto tog = [&](V3 a) { return U2 * a.x + V2 * a.y + N2 * a.z; };
+    V3 io = tog(v3(-hw, hh, -1.0f));
+    V3 idx = tog(v3(2.0f * hw, 0.0f, 0.0f));
+    V3 idy = tog(v3(0.0f, (float)(-2.0 * hh), 0.0f));
+    float vals[18] = {pos[0], pos[1], pos[2], io.x, io.y, io.z, idx.x, idx.y, idx.z,
+                      idy.x, idy.y, idy.z, U2.x, U2.y, U2.z, V2.x, V2.y, V2.z};
+    std::memcpy(out, vals, sizeof(vals));
+}
+
+// Primary rays of one pass, row-major pixel order, 8 floats per ray
+// (org, tnear, dir, tfar), plus the sampler state after the 4 camera draws.
+void orc_raygen(const float* cam, int w, int h, int pass, float* rays, uint32_t* states)
+{
+    Cam c = cam_from(cam);
+    float inv_w = 1.0f / (float)w, inv_h = 1.0f / (float)h;
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            int pid = x + y * w;
+            Sampler s{seed_of(pass, pid)};
+            float u1 = s.next(), u2 = s.next();
+            s.next();
+            s.next();
+            V3 d = cam_dir(c, ((float)x + u1) * inv_w, ((float)y + u2) * inv_h);
+            float* r = rays + 8 * (size_t)pid;
+            r[0] = c.origin.x; r[1] = c.origin.y; r[2] = c.origin.z; r[3] = 0.0f;
+            r[4] = d.x; r[5] = d.y; r[6] = d.z; r[7] = 3.402823466e38f;
+            if (states) states[pid] = s.s;
+        }
+}
+
+void* orc_scene_create(const int32_t* idx, int64_t tri_count, const float* pos, int64_t vertex_count)
+{
+    Scene* sc = new Scene();
+    sc->tris.resize(tri_count);
+    for (int64_t i = 0; i < tri_count; ++i) {
+        const float* a = pos + 3 * (size_t)idx[3 * i];
+        const float* b = pos + 3 * (size_t)idx[3 * i + 1];
+        const float* c = pos + 3 * (size_t)idx[3 * i + 2];
+        sc->tris[i] = Tri{v3(a[0], a[1], a[2]), v3(b[0], b[1], b[2]), v3(c[0], c[1], c[2])};
+    }
+    (void)vertex_count;
+    build_bvh(*sc);
+    return sc;
+}
+
+void orc_scene_destroy(void* sc) { delete static_cast<Scene*>(sc); }
+
+// FP64 closest hit.  rays: 8 floats each; out per ray: prim, t (double),
+// u, v, distance of the hit point to the nearest edge, runner-up prim and t.
+void orc_intersect_fp64(void* scp, const float* rays, int64_t n, int32_t* prim, double* t, double* u,
+                        double* v, double* edge_dist, int32_t* prim2, double* t2, int threads)
+{
+    const Scene& sc = *static_cast<Scene*>(scp);
+    int nt = default_threads(threads);
+    const int64_t chunk = 1024;
+    int ntiles = (int)((n + chunk - 1) / chunk);
+    parallel_tiles(ntiles, nt, [&](int tile) {
+        for (int64_t i = tile * chunk; i < std::min(n, (tile + 1) * chunk); ++i) {
+            const float* r = rays + 8 * (size_t)i;
+            if (r[7] < 0.0f) {
+                prim[i] = -1;
+                t[i] = -1;
+                u[i] = v[i] = 0;
+                edge_dist[i] = INFINITY;
+                if (prim2) prim2[i] = -1;
+                if (t2) t2[i] = INFINITY;
+                continue;
+            }
+            Closest64 c = closest_fp64(sc, v3(r[0], r[1], r[2]), v3(r[4], r[5], r[6]), r[3], r[7], true);
+            prim[i] = c.prim;
+            t[i] = c.prim >= 0 ? c.t : -1.0;
+            u[i] = c.u;
+            v[i] = c.v;
+            edge_dist[i] = c.prim >= 0 ? edge_distance(sc.tris[c.prim], c.u, c.v) : INFINITY;
+            if (prim2) prim2[i] = c.prim2;
+            if (t2) t2[i] = c.t2;
+        }
+    });
+}
+
+// distance between ray i's line and the edges of triangle prim[i] (FP64)
+void orc_ray_edge_distance(void* scp, const float* rays, const int32_t* prims, int64_t n, double* out)
+{
+    const Scene& sc = *static_cast<Scene*>(scp);
+    for (int64_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * (size_t)i;
+        out[i] = prims[i] >= 0 ? ray_edge_distance(D3{r[0], r[1], r[2]}, D3{r[4], r[5], r[6]}, sc.tris[prims[i]])
+                               : INFINITY;
+    }
+}
+
+// kernel-numerics fp32 closest hit by testing every triangle (no BVH);
+// hits as orc_intersect_fp32.  For cross-checking the BVH walk on small scenes.
+void orc_intersect_fp32_brute(void* scp, const float* rays, int64_t n, float* hits)
+{
+    const Scene& sc = *static_cast<Scene*>(scp);
+    for (int64_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * (size_t)i;
+        float* h = hits + 4 * (size_t)i;
+        V3 o = v3(r[0], r[1], r[2]), d = v3(r[4], r[5], r[6]);
+        WRay w = wsetup(d);
+        Closest32 c;
+        c.t = r[7];
+        for (int id = 0; id < (int)sc.tris.size() && r[7] >= 0.0f; ++id) {
+            float t, vn, wn, det;
+            if (!tri_test_f(w, o, sc.tris[id], r[3], c.t, t, vn, wn, det)) continue;
+            if (t < c.t || c.prim < 0 || (t == c.t && id < c.prim)) {
+                c.t = t; c.prim = id; c.u = vn / det; c.v = wn / det;
+            }
+        }
+        int32_t pid = c.prim;
+        if (pid >= 0) { h[0] = refine_t(sc.tris[pid], o, d, c.t); h[2] = c.u; h[3] = c.v; }
+        else { h[0] = -1.0f; h[2] = 0.0f; h[3] = 0.0f; }
+        std::memcpy(&h[1], &pid, 4);
+    }
+}
+
+// kernel-numerics fp32 closest hit; hits: 4 floats (t, prim bits, u, v) each
+void orc_intersect_fp32(void* scp, const float* rays, int64_t n, float* hits, int threads)
+{
+    const Scene& sc = *static_cast<Scene*>(scp);
+    int nt = default_threads(threads);
+    const int64_t chunk = 1024;
+    int ntiles = (int)((n + chunk - 1) / chunk);
+    parallel_tiles(ntiles, nt, [&](int tile) {
+        for (int64_t i = tile * chunk; i < std::min(n, (tile + 1) * chunk); ++i) {
+            const float* r = rays + 8 * (size_t)i;
+            float* h = hits + 4 * (size_t)i;
+            int32_t pid = -1;
+            if (r[7] >= 0.0f) {
+                Closest32 c = closest_fp32(sc, v3(r[0], r[1], r[2]), v3(r[4], r[5], r[6]), r[3], r[7]);
+                if (c.prim >= 0) {
+                    h[0] = refine_t(sc.tris[c.prim], v3(r[0], r[1], r[2]), v3(r[4], r[5], r[6]), c.t);
+                    pid = c.prim;
+                    h[2] = c.u;
+                    h[3] = c.v;
+                }
+            }
+            if (pid < 0) { h[0] = -1.0f; h[2] = 0.0f; h[3] = 0.0f; }
+            std::memcpy(&h[1], &pid, 4);
+        }
+    });
+}
+
+// Single-ray renderer (RendererSingle + DiffuseIntegratorSingle /
+// PrimaryIntegratorSingle), passes pass0 .. pass0+spp-1, 1 sample per pass.
+// accum: 4 floats per pixel (sum r, g, b, weight); sumsq: per-pixel sum of the
+// squared grey value (for the Monte Carlo variance); rays_out: intersect calls.
+// renderer 0 = primary, 1 = diffuse; numerics 0 = kernel fp32, 1 = reference fp64.
+int orc_render(void* scp, const float* cam, int w, int h, int pass0, int spp, int max_depth, float albedo,
+               int renderer, int numerics, int threads, float* accum, float* sumsq, int64_t* rays_out)
+{
+    const Scene& sc = *static_cast<Scene*>(scp);
+    Cam c = cam_from(cam);
+    float inv_w = 1.0f / (float)w, inv_h = 1.0f / (float)h;
+    int nt = default_threads(threads);
+    std::atomic<int64_t> total_rays{0};
+    int rows_per_tile = 8;
+    int ntiles = (h + rows_per_tile - 1) / rows_per_tile;
+    for (int pass = pass0; pass < pass0 + spp; ++pass) {
+        parallel_tiles(ntiles, nt, [&](int tile) {
+            int64_t local_rays = 0;
+            for (int y = tile * rows_per_tile; y < std::min(h, (tile + 1) * rows_per_tile); ++y)
+                for (int x = 0; x < w; ++x) {
+                    int pid = x + y * w;
+                    Sampler s{seed_of(pass, pid)};
+                    float u1 = s.next(), u2 = s.next();
+                    s.next();
+                    s.next();
+                    V3 dir = cam_dir(c, ((float)x + u1) * inv_w, ((float)y + u2) * inv_h);
+                    V3 org = c.origin;
+                    float col[3];
+                    if (renderer == 0) {
+                        HitAny hit = closest_any(sc, org, dir, numerics);
+                        ++local_rays;
+                        if (hit.prim >= 0) {
+                            V3 n = facing_normal(sc.tris[hit.prim], dir);
+                            col[0] = (n.x + 1.0f) * 0.5f;
+                            col[1] = (n.y + 1.0f) * 0.5f;
+                            col[2] = (n.z + 1.0f) * 0.5f;
+                        } else {
+                            col[0] = col[1] = col[2] = 0.05f;
+                        }
+                    } else {
+                        float lw = 1.0f;
+                        int depth = 0;
+                        HitAny hit;
+                        for (;;) {
+                            hit = closest_any(sc, org, dir, numerics);
+                            ++local_rays;
+                            if (hit.prim < 0 || depth == max_depth) break;
+                            lw *= albedo;
+                            V3 n = facing_normal(sc.tris[hit.prim], dir);
+                            float s1 = s.next(), s2 = s.next();
+                            V3 no, nd;
+                            bounce(org, dir, hit.t, n, s1, s2, numerics, no, nd);
+                            org = no;
+                            dir = nd;
+                            ++depth;
+                        }
+                        float L = hit.prim >= 0 ? 0.0f : lw;
+                        col[0] = col[1] = col[2] = L;
+                    }
+                    float* a = accum + 4 * (size_t)pid;
+                    a[0] = a[0] + col[0];
+                    a[1] = a[1] + col[1];
+                    a[2] = a[2] + col[2];
+                    a[3] = a[3] + 1.0f;
+                    if (sumsq) sumsq[pid] += col[0] * col[0];
+                }
+            total_rays += local_rays;
+        });
+    }
+    if (rays_out) *rays_out = total_rays.load();
+    return 0;
+}
+
+// CPU timing baseline: the stream renderer (DiffuseRendererStream) over the
+// oracle's binned-SAH BVH2, fp32 kernel numerics.  Each thread takes 64x32
+// tiles from an atomic counter, generates the tile's rays as one stream,
+// intersects the whole stream, partitions misses before hits, accumulates the
+// misses and shades the hits into the next stream.  Renders `frames` passes
+// of rows [row0, row1); returns rays traced and wall seconds of the frames.
+int orc_stream_bench(void* scp, const float* cam, int w, int h, int row0, int row1, int pass0, int frames,
+                     int max_depth, float albedo, int threads, float* accum, int64_t* rays_out,
+                     double* seconds_out)
+{
+    const Scene& sc = *static_cast<Scene*>(scp);
+    Cam c = cam_from(cam);
+    float inv_w = 1.0f / (float)w, inv_h = 1.0f / (float)h;
+    int nt = default_threads(threads);
+    const int TX = 64, TY = 32;
+    row0 = std::max(0, row0);
+    row1 = std::min(h, row1);
+    int tiles_x = (w + TX - 1) / TX, tiles_y = (row1 - row0 + TY - 1) / TY;
+    std::atomic<int64_t> total{0};
+    auto t0 = std::chrono::steady_clock::now();
+    for (int f = 0; f < frames; ++f) {
+        int pass = pass0 + f;
+        parallel_tiles(tiles_x * tiles_y, nt, [&](int tile) {
+            struct RayS { V3 o, d; float t; int prim; int pix; uint32_t rng; };
+            std::vector<RayS> in, out, hits;
+            in.reserve(TX * TY);
+            int tx = tile % tiles_x, ty = tile / tiles_x;
+            for (int y = row0 + ty * TY; y < std::min(row1, row0 + (ty + 1) * TY); ++y)
+                for (int x = tx * TX; x < std::min(w, (tx + 1) * TX); ++x) {
+                    int pid = x + y * w;
+                    Sampler s{seed_of(pass, pid)};
+                    float u1 = s.next(), u2 = s.next();
+                    s.next();
+                    s.next();
+                    V3 d = cam_dir(c, ((float)x + u1) * inv_w, ((float)y + u2) * inv_h);
+                    in.push_back(RayS{c.origin, d, 0.0f, -1, pid, s.s});
+                }
+            int64_t local = 0;
+            float lw = 1.0f;
+            for (int depth = 0;; ++depth) {
+                for (RayS& r : in) {
+                    Closest32 hc = closest_fp32(sc, r.o, r.d, 0.0f, 3.402823466e38f);
+                    r.prim = hc.prim;
+                    r.t = hc.prim >= 0 ? refine_t(sc.tris[hc.prim], r.o, r.d, hc.t) : hc.t;
+                }
+                local += (int64_t)in.size();
+                // misses first (rayStreamSort), then hits
+                auto mid = std::stable_partition(in.begin(), in.end(), [](const RayS& r) { return r.prim < 0; });
+                for (auto it = in.begin(); it != mid; ++it) {
+                    float* a = accum + 4 * (size_t)it->pix;
+                    a[0] += lw; a[1] += lw; a[2] += lw; a[3] += 1.0f;
+                }
+                if (mid == in.end()) break;
+                if (depth == max_depth) {
+                    for (auto it = mid; it != in.end(); ++it) accum[4 * (size_t)it->pix + 3] += 1.0f;
+                    break;
+                }
+                out.clear();
+                for (auto it = mid; it != in.end(); ++it) {
+                    Sampler s{it->rng};
+                    float s1 = s.next(), s2 = s.next();
+                    V3 n = facing_normal(sc.tris[it->prim], it->d);
+                    V3 no, nd;
+                    bounce(it->o, it->d, it->t, n, s1, s2, 0, no, nd);
+                    out.push_back(RayS{no, nd, 0.0f, -1, it->pix, s.s});
+                }
+                lw *= albedo;
+                std::swap(in, out);
+            }
+            total += local;
+        });
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (rays_out) *rays_out = total.load();
+    if (seconds_out) *seconds_out = std::chrono::duration<double>(t1 - t0).count();
+    return 0;
+}
+
+// Diffuse bounce of n rays: in[i] = org(3), dir(3), t, normal(3), s1, s2
+// (12 floats); out[i] = new org(3), new dir(3).
+void orc_bounce(const float* in, int64_t n, int numerics, float* out)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        const float* a = in + 12 * (size_t)i;
+        V3 no, nd;
+        bounce(v3(a[0], a[1], a[2]), v3(a[3], a[4], a[5]), a[6], v3(a[7], a[8], a[9]), a[10], a[11], numerics,
+               no, nd);
+        float* o = out + 6 * (size_t)i;
+        o[0] = no.x; o[1] = no.y; o[2] = no.z;
+        o[3] = nd.x; o[4] = nd.y; o[5] = nd.z;
+    }
+}
+
+int orc_thread_count(void) { return default_threads(0); }
+
+}  // extern "C"
