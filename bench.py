@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark of the diffuseHip path (ProtoRay `render -r diffuse` on the HIP device).
+
+Workload: 256 icospheres (subdivision 4, 1,310,722 fp32 triangles with the
+ground quad), 1920x1080, max depth 8, albedo 0.7.  One step is one frame: one
+sample per pixel through raygen -> (closest hit -> shade) x (max depth + 1),
+exactly one call of Device::render in the reference
+(render/device_cuda.cpp:95-109).  The metric is the reference's Mrays/s: every
+ray handed to the closest-hit query, primary and secondary, divided by the
+timed seconds (render/diffuse_renderer_cuda.cu:186).
+
+Prints ONE JSON line.  `value` is wall-clock Mrays/s over the K timed frames
+with the scene, BVH and queues resident in HBM.  `roofline` is for the
+dominant kernel (the closest-hit traversal): algorithmic bytes per ray
+48 + 80 * nodes/ray + 36 * tris/ray (ray in, hit out, BVH8 nodes and triangle
+records fetched), nodes/tris per ray from an instrumented replay of the same
+frames, divided by the traversal launches' device time (HIP events on the
+library's stream).  `cpu_baseline` is the oracle's restatement of the CPU
+stream renderer (render/diffuse_renderer_stream.h) on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = "embree-benchmark-protoray_amd"
+METRIC = "Mrays/s diffuse PT 1080p 1.3M tris: plain vs batched; adjoint Mrays/s; %HBM"
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "spheres_1080p"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def wavefront_bytes(rays: int, cont: int, paths: int, n_node: float, n_tri: float) -> float:
+    """Algorithmic HBM bytes of one frame of the wavefront pipeline (per-depth
+    launches with ray queues in HBM): traversal 48 + 80 n_node + 36 n_tri per
+    ray, shading reads 56 per ray, 88 per continuing hit (index, vertices,
+    next-queue write), 32 per finished path (accumulator read + write) and 40
+    per primary ray written by raygen."""
+    return rays * (48 + 56 + 80 * n_node + 36 * n_tri) + cont * 88 + paths * 32 + paths * 40
+
+
+def fused_bytes(rays: int, paths: int, n_node: float, n_tri: float) -> float:
+    """Algorithmic HBM bytes of one frame of the fused pipeline, whose rays,
+    hits and path state never leave the lanes: BVH8 nodes (80 B) and triangle
+    records (36 B) fetched by the traversal, and the accumulator read + write
+    (32 B) of every path."""
+    return rays * (80 * n_node + 36 * n_tri) + paths * 32
+
+
+def cpu_baseline(scenes, mesh, cfg, cam, frames: int) -> dict:
+    sys.path.insert(0, ROOT)
+    from oracle import pyoracle as orc
+    threads = max(1, min(16, os.cpu_count() or 1))
+    sc = orc.Scene(mesh.idx, mesh.pos)
+    w, h = cfg["width"], cfg["height"]
+    r = sc.stream_bench(cam, w, h, 0, h, frames, cfg["max_depth"], cfg["albedo"], threads=threads)
+    return {"value": round(r["rays"] / r["seconds"] / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{frames} full {w}x{h} frames (passes 0-{frames - 1}), max depth {cfg['max_depth']}, "
+                      f"restated stream renderer over a scalar binned-SAH BVH2 (not Embree), "
+                      f"{r['rays']} rays in {r['seconds']:.2f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-frames", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is "
+              "described in DESIGN.md and not built", file=sys.stderr)
+        return 2
+
+    sys.path.insert(0, ROOT)
+    pkg = importlib.import_module(PKG)
+    scenes = importlib.import_module(PKG + ".scenes")
+    cfg = scenes.CONFIGS[WORKLOAD]
+    w, h, depth, albedo = cfg["width"], cfg["height"], cfg["max_depth"], cfg["albedo"]
+    mesh = scenes.build_scene(WORKLOAD)
+    v = cfg["view"]
+    cam = pkg.camera_from_view(v["pos"], v["angle_x"], v["angle_y"], v["fov_y"], w / h)
+
+    dev = pkg.HipDevice(0)
+    log(f"device: {dev.info()}; workload {WORKLOAD}: {mesh.tri_count} triangles, {w}x{h}, depth {depth}")
+    dev.upload_scene(mesh.idx, mesh.pos)
+    build = dev.build()
+    log(f"BVH8 build: {build['buildMs']:.2f} ms, {build['buildMprim']:.1f} Mprim/s, "
+        f"{build['nodes']} nodes, depth {build['depth']}")
+    dev.set_camera(cam)
+    dev.init_frame(w, h)
+
+    for _ in range(args.warmup):
+        dev.render("diffuseHip", max_depth=depth, albedo=albedo)
+    log(f"warmup done ({args.warmup} frames)")
+
+    t0 = time.perf_counter()
+    stats = [dev.render("diffuseHip", max_depth=depth, albedo=albedo) for _ in range(args.steps)]
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    rays = sum(s["ray"] for s in stats)
+    value = rays / wall / 1e6
+    dev_ms = sum(s["renderMs"] for s in stats)
+    log(f"timed {args.steps} frames: {rays} rays in {wall * 1e3:.1f} ms wall ({dev_ms:.1f} ms device) "
+        f"-> {value:.1f} Mrays/s")
+
+    # replay of the same passes: per-launch device time, then traversal counters
+    def replay(timing: bool, counters: bool) -> list:
+        dev.set_pass(args.warmup)
+        dev.set_profiling(timing=timing, counters=counters)
+        r = [dev.render("diffuseHip", max_depth=depth, albedo=albedo) for _ in range(args.steps)]
+        dev.set_profiling()
+        return r
+
+    tstats = replay(True, False)
+    cstats = replay(False, True)
+    kernel_ms = sum(s["traceMs"] for s in tstats)
+    crays = max(1, sum(s["ray"] for s in cstats))
+    n_node = sum(s["nodes"] for s in cstats) / crays
+    n_tri = sum(s["tris"] for s in cstats) / crays
+    paths = sum(s["paths"] for s in stats)
+    cont = rays - paths
+    kbytes = fused_bytes(rays, paths, n_node, n_tri)
+    achieved = kbytes / (kernel_ms * 1e-3) / 1e9
+    survey_bytes = wavefront_bytes(rays, cont, paths, n_node, n_tri)
+    depth_rays = [sum(s["queue"][k] for s in cstats) // args.steps for k in range(depth + 1)]
+    log(f"fused path kernel: {n_node:.2f} nodes/ray, {n_tri:.2f} tris/ray, {kernel_ms:.1f} ms over "
+        f"{args.steps} launches ({achieved:.0f} GB/s algorithmic); rays per depth {depth_rays}")
+
+    # the reference-structured wavefront pipeline on the same frames, for comparison
+    dev.set_pipeline("wavefront")
+    dev.set_pass(args.warmup)
+    t2 = time.perf_counter()
+    wstats = [dev.render("diffuseHip", max_depth=depth, albedo=albedo) for _ in range(args.steps)]
+    wwall = time.perf_counter() - t2
+    dev.set_pipeline("fused")
+    wrays = sum(s["ray"] for s in wstats)
+    assert wrays == rays, "wavefront and fused pipelines traced different ray counts"
+    log(f"wavefront pipeline: {wrays / wwall / 1e6:.1f} Mrays/s")
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "spheres_1080p: 256 icospheres (subdiv 4) + ground quad, 1,310,722 tris, "
+                               "1920x1080, 1 spp per step, max depth 8, albedo 0.7 (diffuseHip, plain)",
+                   "triangles": mesh.tri_count, "width": w, "height": h, "max_depth": depth, "albedo": albedo,
+                   "pipeline": "fused"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_trace<fused diffuse> (persistent path kernel: BVH8 closest hit + shading)",
+                     "bytes_per_ray": round(kbytes / rays, 1),
+                     "avg_launch_ms": round(kernel_ms / args.steps, 4)},
+        "device_mrays": round(rays / (dev_ms * 1e-3) / 1e6, 2),
+        "wavefront_mrays": round(wrays / wwall / 1e6, 2),
+        "frame_hbm_frac_wavefront_formula": round(survey_bytes / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "nodes_per_ray": round(n_node, 3), "tris_per_ray": round(n_tri, 3),
+        "rays_per_frame": rays // args.steps, "rays_per_depth": depth_rays,
+        "build": {"buildMs": round(build["buildMs"], 3), "buildMprim": round(build["buildMprim"], 2)},
+        "cpu_baseline": None,
+    }
+    dev.close()
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(scenes, mesh, cfg, cam, args.cpu_frames)
+        log(f"cpu baseline: {out['cpu_baseline']['value']} Mrays/s on {out['cpu_baseline']['cores']} threads")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -44,7 +44,8 @@ class PrtBuildStats(ctypes.Structure):
 
 class PrtRenderStats(ctypes.Structure):
     _fields_ = [("rays", ctypes.c_int64), ("ms", ctypes.c_double), ("mray", ctypes.c_double),
-                ("paths", ctypes.c_int64), ("queue", ctypes.c_int64 * 16)]
+                ("paths", ctypes.c_int64), ("queue", ctypes.c_int64 * 16), ("trace_ms", ctypes.c_double),
+                ("shade_ms", ctypes.c_double), ("nodes", ctypes.c_int64), ("tris", ctypes.c_int64)]
 
 
 class PrtTraceCounters(ctypes.Structure):
@@ -90,6 +91,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "prt_hip_render": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.POINTER(PrtRenderStats)],
                            ctypes.c_int),
         "prt_hip_set_pass": ([P, ctypes.c_int], ctypes.c_int),
+        "prt_hip_set_profiling": ([P, ctypes.c_int], ctypes.c_int),
+        "prt_hip_set_pipeline": ([P, ctypes.c_int], ctypes.c_int),
         "prt_hip_accum_read": ([P, P], ctypes.c_int),
         "prt_hip_frame_read_bgr8": ([P, P], ctypes.c_int),
         "prt_hip_debug_raygen": ([P, ctypes.c_int, P, P], ctypes.c_int),
@@ -239,7 +242,19 @@ class HipDevice:
         st = PrtRenderStats()
         self._check(self.lib.prt_hip_render(self.ctx, renderer, max_depth, albedo, ctypes.byref(st)), "render")
         return {"ray": st.rays, "renderMs": st.ms, "mray": st.mray, "paths": st.paths,
-                "queue": [st.queue[k] for k in range(16)]}
+                "queue": [st.queue[k] for k in range(16)], "traceMs": st.trace_ms, "shadeMs": st.shade_ms,
+                "nodes": st.nodes, "tris": st.tris}
+
+    def set_pipeline(self, pipeline: str) -> None:
+        """'fused' (one persistent path launch per frame) or 'wavefront' (per-depth launches)."""
+        ids = {"fused": 0, "wavefront": 1}
+        if pipeline not in ids:
+            raise ValueError("pipeline must be 'fused' or 'wavefront'")
+        self._check(self.lib.prt_hip_set_pipeline(self.ctx, ids[pipeline]), "set_pipeline")
+
+    def set_profiling(self, timing: bool = False, counters: bool = False) -> None:
+        flags = (1 if timing else 0) | (2 if counters else 0)
+        self._check(self.lib.prt_hip_set_profiling(self.ctx, flags), "set_profiling")
 
     def accum(self) -> np.ndarray:
         out = np.zeros((self.height, self.width, 4), dtype=np.float32)

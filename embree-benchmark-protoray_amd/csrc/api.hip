@@ -138,12 +138,12 @@ int prt_hip_create(int device_index, prt_hip_ctx** out_ctx)
         hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
         hipMalloc(&ctx->d_counters, 64 * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&ctx->d_stats, 8 * sizeof(unsigned long long)) != hipSuccess) {
+        hipMalloc(&ctx->d_stats, 32 * sizeof(unsigned long long)) != hipSuccess) {
         prt_hip_destroy(ctx);
         return 5;
     }
     ctx->num_cus = prop.multiProcessorCount;
-    ctx->device_name = prop.name;
+    ctx->device_name = prop.name[0] ? std::string(prop.name) : std::string("AMD ") + prop.gcnArchName;
     if (hipMemsetAsync(ctx->d_counters, 0, 64 * sizeof(uint32_t), ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess) {
         prt_hip_destroy(ctx);
@@ -163,6 +163,8 @@ int prt_hip_destroy(prt_hip_ctx* ctx)
                     ctx->d_pix[0], ctx->d_pix[1], ctx->d_rng[0], ctx->d_rng[1], ctx->d_counters,
                     ctx->d_stats, ctx->d_io_rays, ctx->d_io_hits};
     for (void* p : ptrs) free_ptr(p);
+    for (hipEvent_t e : ctx->kev)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -260,10 +262,9 @@ int prt_hip_intersect_counters(prt_hip_ctx* ctx, const prt_ray* rays, prt_hit* h
     return check_trace_error(ctx);
 }
 
-int prt_hip_camera_from_view(const float position[3], float angle_x, float angle_y, float fov_y,
-                             float aspect_ratio, prt_camera* out)
+int prt_hip_view_basis(float angle_x, float angle_y, float out_basis[9])
 {
-    if (!position || !out) return 1;
+    if (!out_basis) return 1;
     // rotate(vec, axis, a) of math/vec3.h:409-412
     auto rot = [](f3 vec, f3 axis, float a) {
         float c = std::cos(a), s = std::sin(a);
@@ -271,19 +272,34 @@ int prt_hip_camera_from_view(const float position[3], float angle_x, float angle
         return add3(r, mul3(cross3(vec, axis), s));
     };
     const f3 V = mk3(0.0f, 1.0f, 0.0f), N = mk3(0.0f, 0.0f, 1.0f);
-    // rotateV(angleY), then rotateU(angleX) (math/basis3.h:63-79)
+    // Basis3f(one).rotateV(angleY).rotateU(angleX) (math/basis3.h:63-79)
     const f3 n1 = rot(N, V, angle_y);
     const f3 u1 = cross3(V, n1);
     const f3 v2 = rot(V, u1, angle_x);
     const f3 n2 = cross3(u1, v2);
-    const f3 u2 = u1;
+    const f3 b[3] = {u1, v2, n2};
+    for (int k = 0; k < 3; ++k) {
+        out_basis[3 * k + 0] = b[k].x;
+        out_basis[3 * k + 1] = b[k].y;
+        out_basis[3 * k + 2] = b[k].z;
+    }
+    return 0;
+}
+
+int prt_hip_camera_from_basis(const float position[3], const float basis[9], float fov_y, float aspect_ratio,
+                              prt_camera* out)
+{
+    if (!position || !basis || !out) return 1;
+    const f3 u = mk3(basis[0], basis[1], basis[2]);
+    const f3 v = mk3(basis[3], basis[4], basis[5]);
+    const f3 n = mk3(basis[6], basis[7], basis[8]);
     const float hh = std::tan(0.5f * fov_y);
     const float hw = hh * aspect_ratio;
-    auto to_global = [&](f3 a) { return add3(add3(mul3(u2, a.x), mul3(v2, a.y)), mul3(n2, a.z)); };
+    auto to_global = [&](f3 a) { return add3(add3(mul3(u, a.x), mul3(v, a.y)), mul3(n, a.z)); };
     const f3 io = to_global(mk3(-hw, hh, -1.0f));
     const f3 dx = to_global(mk3(2.0f * hw, 0.0f, 0.0f));
     const f3 dy = to_global(mk3(0.0f, (float)(-2.0 * (double)hh), 0.0f));
-    const f3 vals[6] = {mk3(position[0], position[1], position[2]), io, dx, dy, u2, v2};
+    const f3 vals[6] = {mk3(position[0], position[1], position[2]), io, dx, dy, u, v};
     float* dst[6] = {out->origin, out->image_o, out->image_dx, out->image_dy, out->basis_u, out->basis_v};
     for (int k = 0; k < 6; ++k) {
         dst[k][0] = vals[k].x;
@@ -293,6 +309,14 @@ int prt_hip_camera_from_view(const float position[3], float angle_x, float angle
     out->lens_radius = 0.0f;
     out->focal_distance = 1.0f;
     return 0;
+}
+
+int prt_hip_camera_from_view(const float position[3], float angle_x, float angle_y, float fov_y,
+                             float aspect_ratio, prt_camera* out)
+{
+    float basis[9];
+    if (!position || !out || prt_hip_view_basis(angle_x, angle_y, basis)) return 1;
+    return prt_hip_camera_from_basis(position, basis, fov_y, aspect_ratio, out);
 }
 
 int prt_hip_camera_set(prt_hip_ctx* ctx, const prt_camera* camera)
@@ -339,6 +363,26 @@ int prt_hip_set_pass(prt_hip_ctx* ctx, int pass)
     return 0;
 }
 
+int prt_hip_set_profiling(prt_hip_ctx* ctx, int flags)
+{
+    if (use_device(ctx)) return 1;
+    if (flags & PRT_PROFILE_TIMING) {
+        for (hipEvent_t& e : ctx->kev)
+            if (!e) PRT_TRY(ctx, hipEventCreate(&e));
+    }
+    ctx->profiling = flags;
+    return 0;
+}
+
+int prt_hip_set_pipeline(prt_hip_ctx* ctx, int pipeline)
+{
+    if (!ctx) return 1;
+    if (pipeline != PRT_PIPELINE_FUSED && pipeline != PRT_PIPELINE_WAVEFRONT)
+        return set_error(ctx, "set_pipeline: unknown pipeline");
+    ctx->pipeline = pipeline;
+    return 0;
+}
+
 int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, prt_render_stats* out)
 {
     if (use_device(ctx)) return 1;
@@ -351,29 +395,62 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
         return set_error(ctx, "render: max_depth must be in [0, 14]");
     if (ensure_trace_resources(ctx)) return 1;
 
-    PRT_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    PRT_TRY(ctx, launch_raygen(ctx, ctx->pass, ctx->d_rays[0], ctx->d_pix[0], ctx->d_rng[0]));
+    const bool timing = (ctx->profiling & PRT_PROFILE_TIMING) != 0;
+    const bool counting = (ctx->profiling & PRT_PROFILE_COUNTERS) != 0;
+    const bool fused = ctx->pipeline == PRT_PIPELINE_FUSED;
     const int last = renderer == PRT_RENDERER_PRIMARY ? 0 : max_depth;
-    float throughput = 1.0f;
-    for (int depth = 0; depth <= last; ++depth) {
-        TraceLaunch t{};
-        t.rays = ctx->d_rays[depth & 1];
-        t.hits = ctx->d_hits;
-        t.count_ptr = ctx->d_counters + depth;
-        t.count = 0;
-        t.fetch_ptr = ctx->d_counters + 16 + depth;
-        t.stats = nullptr;
-        PRT_TRY(ctx, launch_trace(ctx, t, false));
-        if (renderer == PRT_RENDERER_PRIMARY) {
-            PRT_TRY(ctx, launch_shade_primary(ctx));
-        } else {
-            PRT_TRY(ctx, launch_shade_diffuse(ctx, depth, max_depth, throughput, albedo));
-            throughput *= albedo;
+    const int tiles_x = (ctx->width + 7) / 8;
+    const uint32_t entries = (uint32_t)frame_entries(ctx);
+
+    PRT_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (fused) {
+        PRT_TRY(ctx, launch_frame_begin(ctx, entries));
+        PathLaunch p{};
+        p.cam = to_camera(ctx->cam);
+        p.width = ctx->width;
+        p.height = ctx->height;
+        p.tiles_x = tiles_x;
+        p.entries = entries;
+        p.pass = ctx->pass;
+        p.inv_w = 1.0f / (float)ctx->width;
+        p.inv_h = 1.0f / (float)ctx->height;
+        p.accum = ctx->d_accum;
+        p.max_depth = last;
+        p.albedo = albedo;
+        p.path_counter = ctx->d_counters + 16;
+        p.depth_rays = ctx->d_stats + 16;
+        p.stats = ctx->d_stats;
+        if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[0], ctx->stream));
+        PRT_TRY(ctx, launch_paths(ctx, p, renderer == PRT_RENDERER_PRIMARY, counting));
+        if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[1], ctx->stream));
+    } else {
+        PRT_TRY(ctx, launch_raygen(ctx, ctx->pass, ctx->d_rays[0], ctx->d_pix[0], ctx->d_rng[0]));
+        float throughput = 1.0f;
+        for (int depth = 0; depth <= last; ++depth) {
+            TraceLaunch t{};
+            t.rays = ctx->d_rays[depth & 1];
+            t.hits = ctx->d_hits;
+            t.count_ptr = ctx->d_counters + depth;
+            t.count = 0;
+            t.fetch_ptr = ctx->d_counters + 16 + depth;
+            t.stats = counting ? ctx->d_stats : nullptr;
+            if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[4 * depth + 0], ctx->stream));
+            PRT_TRY(ctx, launch_trace(ctx, t, counting));
+            if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[4 * depth + 1], ctx->stream));
+            if (renderer == PRT_RENDERER_PRIMARY) {
+                PRT_TRY(ctx, launch_shade_primary(ctx));
+            } else {
+                PRT_TRY(ctx, launch_shade_diffuse(ctx, depth, max_depth, throughput, albedo));
+                throughput *= albedo;
+            }
+            if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[4 * depth + 2], ctx->stream));
         }
     }
     PRT_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     uint32_t counters[33];
+    unsigned long long st[32];
     PRT_TRY(ctx, hipMemcpyAsync(counters, ctx->d_counters, sizeof(counters), hipMemcpyDeviceToHost, ctx->stream));
+    PRT_TRY(ctx, hipMemcpyAsync(st, ctx->d_stats, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
     PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (counters[32]) return set_error(ctx, "render: traversal stack overflow");
     float ms = 0.0f;
@@ -382,16 +459,36 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
     if (out) {
         std::memset(out, 0, sizeof(*out));
         const int64_t live = (int64_t)ctx->width * ctx->height;
-        int64_t rays = live;
-        out->queue[0] = live;
-        for (int k = 1; k <= last; ++k) {
-            out->queue[k] = counters[k];
-            rays += counters[k];
+        int64_t rays = 0;
+        for (int k = 0; k <= last; ++k) {
+            const int64_t q = fused ? (int64_t)st[16 + k] : (k == 0 ? live : (int64_t)counters[k]);
+            out->queue[k] = q;
+            rays += q;
         }
+        if (fused) rays = (int64_t)st[3];
         out->rays = rays;
         out->paths = live;
         out->ms = ms;
         out->mray = ms > 0.0f ? (double)rays / 1000.0 / ms : 0.0;
+        if (timing) {
+            if (fused) {
+                float a = 0.0f;
+                PRT_TRY(ctx, hipEventElapsedTime(&a, ctx->kev[0], ctx->kev[1]));
+                out->trace_ms = a;
+            } else {
+                for (int depth = 0; depth <= last; ++depth) {
+                    float a = 0.0f, b = 0.0f;
+                    PRT_TRY(ctx, hipEventElapsedTime(&a, ctx->kev[4 * depth + 0], ctx->kev[4 * depth + 1]));
+                    PRT_TRY(ctx, hipEventElapsedTime(&b, ctx->kev[4 * depth + 1], ctx->kev[4 * depth + 2]));
+                    out->trace_ms += a;
+                    out->shade_ms += b;
+                }
+            }
+        }
+        if (counting) {
+            out->nodes = (int64_t)st[0];
+            out->tris = (int64_t)st[1];
+        }
     }
     return 0;
 }

@@ -96,6 +96,9 @@ struct prt_hip_ctx {
     int64_t io_cap = 0;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int profiling = 0;
+    int pipeline = PRT_PIPELINE_FUSED;
+    hipEvent_t kev[4 * 16] = {};    // per-launch timing events (trace/shade x begin/end)
 };
 
 namespace prt {
@@ -112,12 +115,29 @@ struct TraceLaunch {
     uint32_t* fetch_ptr;        // device fetch counter, zero before launch
     unsigned long long* stats;  // nullptr unless instrumented
 };
+// Fused path launch (one frame, one sample per pixel)
+struct PathLaunch {
+    Camera cam;
+    int width, height, tiles_x;
+    uint32_t entries;                // tiles * 64 path slots (partial tiles hold dead slots)
+    int pass;
+    float inv_w, inv_h;
+    float4* accum;
+    int max_depth;
+    float albedo;
+    uint32_t* path_counter;          // zero before launch
+    unsigned long long* depth_rays;  // [16] rays traced per depth, zero before launch
+    unsigned long long* stats;       // instrumented counters
+};
 int ensure_trace_resources(prt_hip_ctx* ctx);
 hipError_t launch_trace(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumented);
+hipError_t launch_paths(prt_hip_ctx* ctx, const PathLaunch& p, bool primary, bool instrumented);
 
 // render.hip
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t s);
 hipError_t launch_raygen(prt_hip_ctx* ctx, int pass, float4* rays, int32_t* pix, uint32_t* rng);
+hipError_t launch_frame_begin(prt_hip_ctx* ctx, uint32_t entries);
+Camera to_camera(const prt_camera& c);
 hipError_t launch_shade_diffuse(prt_hip_ctx* ctx, int depth, int max_depth, float throughput,
                                 float albedo_unused);
 hipError_t launch_shade_primary(prt_hip_ctx* ctx);

@@ -59,9 +59,11 @@ __global__ __launch_bounds__(kBlock) void k_raygen(Camera cam, int width, int he
     rng[gid] = s;
 }
 
-__global__ __launch_bounds__(kBlock) void k_frame_begin(uint32_t* counters, uint32_t n0)
+__global__ __launch_bounds__(kBlock) void k_frame_begin(uint32_t* counters, uint32_t n0,
+                                                        unsigned long long* stats)
 {
     if (threadIdx.x < 33) counters[threadIdx.x] = threadIdx.x == 0 ? n0 : 0u;
+    if (threadIdx.x < 32) stats[threadIdx.x] = 0ull;
 }
 
 __device__ __forceinline__ void tri_vertices(const int32_t* idx, const float* pos, int prim, f3& p0,
@@ -217,7 +219,7 @@ hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t s)
     return hipGetLastError();
 }
 
-static Camera to_camera(const prt_camera& c)
+Camera to_camera(const prt_camera& c)
 {
     Camera k;
     k.origin = mk3(c.origin[0], c.origin[1], c.origin[2]);
@@ -232,10 +234,16 @@ hipError_t launch_raygen(prt_hip_ctx* ctx, int pass, float4* rays, int32_t* pix,
     const int tiles_x = (ctx->width + 7) / 8, tiles_y = (ctx->height + 7) / 8;
     const int tiles = tiles_x * tiles_y;
     const uint32_t entries = (uint32_t)tiles * 64u;
-    k_frame_begin<<<1, 64, 0, ctx->stream>>>(ctx->d_counters, entries);
+    k_frame_begin<<<1, 64, 0, ctx->stream>>>(ctx->d_counters, entries, ctx->d_stats);
     const float inv_w = 1.0f / (float)ctx->width, inv_h = 1.0f / (float)ctx->height;
     k_raygen<<<(tiles * 64 + kBlock - 1) / kBlock, kBlock, 0, ctx->stream>>>(
         to_camera(ctx->cam), ctx->width, ctx->height, tiles_x, tiles, pass, inv_w, inv_h, rays, pix, rng);
+    return hipGetLastError();
+}
+
+hipError_t launch_frame_begin(prt_hip_ctx* ctx, uint32_t entries)
+{
+    k_frame_begin<<<1, 64, 0, ctx->stream>>>(ctx->d_counters, entries, ctx->d_stats);
     return hipGetLastError();
 }
 

@@ -1,21 +1,36 @@
-// trace.hip -- closest-hit traversal of the quantised BVH8.
+// trace.hip -- BVH8 closest-hit traversal, alone or fused with the diffuse
+// path it serves.
 //
 // Replaces IntersectorStreamCuda::intersect (core/intersector_stream_cuda.h:27-34),
 // which the reference forwards to OptiX Prime's closest query
-// (accel/optix/optix_intersector_stream_cuda.cpp:77-82).
+// (accel/optix/optix_intersector_stream_cuda.cpp:77-82), and, in the fused
+// modes, the raygen / shade kernels and the per-bounce host loop of
+// DiffuseRendererCuda::render (render/diffuse_renderer_cuda.cu:27-120,159-212).
 //
-// One ray per lane.  The kernel is persistent: each wave takes rays from a
-// global counter; when at least kRefill of its 64 lanes have finished, the
-// wave ballots the idle lanes, takes that many new rays with one atomic and
-// hands them out by lane prefix (mbcnt), so lanes stay busy on incoherent
-// bounces.  The traversal stack is a per-lane ring of kStackLds entries in
-// LDS; deeper entries spill to a per-lane slice of global memory.  Each entry
-// keeps the entry distance of its box, so nodes that fall behind the closest
-// hit are dropped when popped.
+// One ray per lane.  The kernel is persistent: each wave takes work from a
+// global counter; when at least kRefill of its 64 lanes are idle, the wave
+// ballots them, takes that many work items with one atomic and hands them out
+// by lane prefix, so lanes stay busy on incoherent bounces.  The traversal
+// stack is a per-lane ring of kStackLds entries in LDS; deeper entries spill
+// to a per-lane slice of global memory.  Each entry keeps the entry distance
+// of its box, so nodes behind the closest hit are dropped when popped.
 //
-// Result: the hit with the smallest (t, primId) among triangles whose
-// watertight test accepts tnear <= t <= tfar; the same answer as testing every
-// triangle, independent of the visiting order.
+// Modes:
+//   kModeTrace    work item = a ray of the input array; writes its hit record.
+//   kModeDiffuse  work item = a path (one pixel sample).  The lane generates
+//                 the camera ray, and when a ray finishes it shades it in
+//                 place: a miss adds the path throughput, a hit at max depth
+//                 adds 0, any other hit bounces (cosine hemisphere) and the
+//                 lane traces the next ray of the same path.  Paths of every
+//                 depth share the launch, so a frame pays the latency of its
+//                 slowest path once instead of once per bounce, and no ray or
+//                 hit record goes through HBM.
+//   kModePrimary  as kModeDiffuse with the primary shading ((N+1)/2, 0.05).
+//
+// Result of a query: the hit with the smallest (t, primId) among triangles
+// whose watertight test accepts tnear <= t <= tfar -- the same answer as
+// testing every triangle, whatever the visiting order -- with t then refined
+// in double against the winner's plane.
 #include "prt_internal.h"
 
 namespace prt {
@@ -24,6 +39,8 @@ namespace {
 constexpr int kStackLds = 16;      // LDS entries per lane (power of two)
 constexpr int kRefill = 24;        // refill when this many lanes are idle
 
+enum { kModeTrace = 0, kModeDiffuse = 1, kModePrimary = 2 };
+
 __device__ __forceinline__ float byte_f(uint32_t w, int k)
 {
     return (float)((w >> (8 * k)) & 0xffu);
@@ -31,19 +48,31 @@ __device__ __forceinline__ float byte_f(uint32_t w, int k)
 
 __device__ __forceinline__ float as_f(uint32_t u) { return __uint_as_float(u); }
 
-template <bool kCount>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaunch tl, uint2* spill,
-                                                       int stack_entries, uint32_t* err)
+template <bool kCount, int kMode>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaunch tl, PathLaunch pl,
+                                                       uint2* spill, int stack_entries, uint32_t* err)
 {
     __shared__ uint32_t st_node[kStackLds * kTraceBlock];
     __shared__ float st_t[kStackLds * kTraceBlock];
+    // rays per depth: an LDS histogram in the instrumented variant only, so
+    // the timed variant keeps 32 KiB of LDS per block (5 blocks per CU)
+    uint32_t* depth_hist = nullptr;
+    if constexpr (kCount && kMode != kModeTrace) {
+        __shared__ uint32_t hist[kMaxQueueDepth];
+        depth_hist = hist;
+    }
+    uint32_t n_rays = 0;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const uint64_t lane_lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t count = tl.count_ptr ? *tl.count_ptr : tl.count;
+    const uint32_t count = kMode == kModeTrace ? (tl.count_ptr ? *tl.count_ptr : tl.count) : pl.entries;
     uint2* my_spill = spill + ((size_t)blockIdx.x * kTraceBlock + tid) * (size_t)stack_entries;
     const int spill_cap = stack_entries;
+    if constexpr (kCount && kMode != kModeTrace) {
+        if (tid < kMaxQueueDepth) depth_hist[tid] = 0;
+        __syncthreads();
+    }
 
     bool active = false;
     uint32_t ray_idx = 0;
@@ -57,46 +86,79 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
     float bu = 0.0f, bv = 0.0f;
     int sp = 0, bottom = 0;
     int32_t node = -1;
+    // path state (fused modes)
+    int32_t pixel = 0;
+    uint32_t rng = 0;
+    int depth = 0;
+    float lw = 1.0f;
     unsigned long long n_nodes = 0, n_tris = 0, n_spill = 0;
-    bool exhausted = false;  // wave-uniform: the ray counter has run past count
+    bool exhausted = false;  // wave-uniform: the work counter has run past count
+
+    auto start_ray = [&](f3 org, f3 dir, float near, float far) {
+        o = org;
+        d = dir;
+        tmin = near;
+        tbest = far;
+        const float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
+        const float dy = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
+        const float dz = fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z;
+        inv_x = __builtin_amdgcn_rcpf(dx);
+        inv_y = __builtin_amdgcn_rcpf(dy);
+        inv_z = __builtin_amdgcn_rcpf(dz);
+        neg_x = dx < 0.0f;
+        neg_y = dy < 0.0f;
+        neg_z = dz < 0.0f;
+        wr = watertight_setup(d);
+        best_prim = -1;
+        bu = bv = 0.0f;
+        sp = bottom = 0;
+        node = 0;
+    };
 
     for (;;) {
         // ---- refill idle lanes (wave-uniform branch)
-        uint64_t idle = __ballot(!active);
-        int n_idle = __popcll(idle);
+        const uint64_t idle = __ballot(!active);
+        const int n_idle = __popcll(idle);
         if (!exhausted && n_idle >= kRefill) {
-            int leader = __ffsll((unsigned long long)idle) - 1;
+            const int leader = __ffsll((unsigned long long)idle) - 1;
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(tl.fetch_ptr, (uint32_t)n_idle);
+            if (lane == leader) base = atomicAdd(kMode == kModeTrace ? tl.fetch_ptr : pl.path_counter,
+                                                 (uint32_t)n_idle);
             base = __shfl(base, leader, 64);
             if (!active) {
-                uint32_t my = base + (uint32_t)__popcll(idle & lane_lt);
+                const uint32_t my = base + (uint32_t)__popcll(idle & lane_lt);
                 if (my < count) {
-                    ray_idx = my;
-                    float4 r0 = tl.rays[2 * (size_t)my];
-                    float4 r1 = tl.rays[2 * (size_t)my + 1];
-                    o = mk3(r0.x, r0.y, r0.z);
-                    tmin = r0.w;
-                    d = mk3(r1.x, r1.y, r1.z);
-                    tbest = r1.w;
-                    if (tbest < 0.0f) {
-                        tl.hits[my] = make_float4(-1.0f, __int_as_float(-1), 0.0f, 0.0f);
+                    if (kMode == kModeTrace) {
+                        ray_idx = my;
+                        const float4 r0 = tl.rays[2 * (size_t)my];
+                        const float4 r1 = tl.rays[2 * (size_t)my + 1];
+                        if (r1.w < 0.0f) {
+                            tl.hits[my] = make_float4(-1.0f, __int_as_float(-1), 0.0f, 0.0f);
+                        } else {
+                            start_ray(mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), r0.w, r1.w);
+                            active = true;
+                        }
                     } else {
-                        active = true;
-                        float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
-                        float dy = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
-                        float dz = fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z;
-                        inv_x = __builtin_amdgcn_rcpf(dx);
-                        inv_y = __builtin_amdgcn_rcpf(dy);
-                        inv_z = __builtin_amdgcn_rcpf(dz);
-                        neg_x = dx < 0.0f;
-                        neg_y = dy < 0.0f;
-                        neg_z = dz < 0.0f;
-                        wr = watertight_setup(d);
-                        best_prim = -1;
-                        bu = bv = 0.0f;
-                        sp = bottom = 0;
-                        node = 0;
+                        // camera ray of path `my`: 8x8 tiles, Morton order inside
+                        // a tile (image/morton.cpp:22), as the wavefront raygen
+                        const uint32_t tile = my >> 6, tl_lane = my & 63u;
+                        const int mx = (tl_lane & 1) | ((tl_lane >> 1) & 2) | ((tl_lane >> 2) & 4);
+                        const int my_ = ((tl_lane >> 1) & 1) | ((tl_lane >> 2) & 2) | ((tl_lane >> 3) & 4);
+                        const int x = (int)(tile % (uint32_t)pl.tiles_x) * 8 + mx;
+                        const int y = (int)(tile / (uint32_t)pl.tiles_x) * 8 + my_;
+                        if (x < pl.width && y < pl.height) {
+                            pixel = x + y * pl.width;
+                            rng = sampler_seed(pl.pass, pixel);
+                            const float u1 = sampler_next(rng);
+                            const float u2 = sampler_next(rng);
+                            (void)sampler_next(rng);  // lens 2D
+                            (void)sampler_next(rng);
+                            const f3 dir = camera_dir(pl.cam, ((float)x + u1) * pl.inv_w, ((float)y + u2) * pl.inv_h);
+                            start_ray(pl.cam.origin, dir, 0.0f, 3.402823466e38f);
+                            depth = 0;
+                            lw = 1.0f;
+                            active = true;
+                        }
                     }
                 }
             }
@@ -116,33 +178,71 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
                     uint32_t nid;
                     float nt;
                     if (sp < bottom) {
-                        uint2 e = my_spill[sp];
+                        const uint2 e = my_spill[sp];
                         bottom = sp;
                         nid = e.x;
                         nt = as_f(e.y);
                     } else {
-                        int slot = (sp & (kStackLds - 1)) * kTraceBlock + tid;
+                        const int slot = (sp & (kStackLds - 1)) * kTraceBlock + tid;
                         nid = st_node[slot];
                         nt = st_t[slot];
                     }
                     if (nt <= tbest) { node = (int32_t)nid; break; }
                 }
                 if (node < 0) {
+                    // ---- the ray is finished
+                    float t = -1.0f;
+                    f3 p0{}, p1{}, p2{};
                     if (best_prim >= 0) {
                         const float* tp = sc.tris + 9 * (size_t)best_rec;
-                        const float t = refine_t(mk3(tp[0], tp[1], tp[2]), mk3(tp[3], tp[4], tp[5]),
-                                                 mk3(tp[6], tp[7], tp[8]), o, d, tbest);
-                        tl.hits[ray_idx] = make_float4(t, __int_as_float(best_prim), bu, bv);
-                    } else
-                        tl.hits[ray_idx] = make_float4(-1.0f, __int_as_float(-1), 0.0f, 0.0f);
-                    active = false;
-                    break;
+                        p0 = mk3(tp[0], tp[1], tp[2]);
+                        p1 = mk3(tp[3], tp[4], tp[5]);
+                        p2 = mk3(tp[6], tp[7], tp[8]);
+                        t = refine_t(p0, p1, p2, o, d, tbest);
+                    }
+                    if (kMode == kModeTrace) {
+                        tl.hits[ray_idx] = best_prim >= 0
+                                               ? make_float4(t, __int_as_float(best_prim), bu, bv)
+                                               : make_float4(-1.0f, __int_as_float(-1), 0.0f, 0.0f);
+                        active = false;
+                        break;
+                    }
+                    if constexpr (kCount) atomicAdd(&depth_hist[depth], 1u);
+                    ++n_rays;
+                    float4* acc = pl.accum + pixel;
+                    if (kMode == kModePrimary) {
+                        f3 c = mk3(0.05f, 0.05f, 0.05f);
+                        if (best_prim >= 0) {
+                            const f3 n = face_normal(p0, p1, p2, d);
+                            c = mk3((n.x + 1.0f) * 0.5f, (n.y + 1.0f) * 0.5f, (n.z + 1.0f) * 0.5f);
+                        }
+                        const float4 a = *acc;
+                        *acc = make_float4(a.x + c.x, a.y + c.y, a.z + c.z, a.w + 1.0f);
+                        active = false;
+                        break;
+                    }
+                    if (best_prim < 0 || depth == pl.max_depth) {
+                        const float c = best_prim < 0 ? lw : 0.0f;
+                        const float4 a = *acc;
+                        *acc = make_float4(a.x + c, a.y + c, a.z + c, a.w + 1.0f);
+                        active = false;
+                        break;
+                    }
+                    // diffuse bounce: the next ray of this path
+                    lw *= pl.albedo;
+                    const f3 n = face_normal(p0, p1, p2, d);
+                    const float s1 = sampler_next(rng);
+                    const float s2 = sampler_next(rng);
+                    f3 no, nd;
+                    diffuse_bounce(o, d, t, n, s1, s2, no, nd);
+                    ++depth;
+                    start_ray(no, nd, 0.0f, 3.402823466e38f);
                 }
             }
 
             // ---- visit one BVH8 node
             const uint4* np = reinterpret_cast<const uint4*>(sc.nodes + node);
-            uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
+            const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
             if (kCount) ++n_nodes;
             const int ex = (int)(int8_t)(w0.w & 0xffu);
             const int ey = (int)(int8_t)((w0.w >> 8) & 0xffu);
@@ -178,14 +278,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
                 const uint32_t nxw = k < 4 ? nx0 : nx1, fxw = k < 4 ? fx0 : fx1;
                 const uint32_t nyw = k < 4 ? ny0 : ny1, fyw = k < 4 ? fy0 : fy1;
                 const uint32_t nzw = k < 4 ? nz0 : nz1, fzw = k < 4 ? fz0 : fz1;
-                float tnx = __builtin_fmaf(byte_f(nxw, kb), ax, bxn);
-                float tny = __builtin_fmaf(byte_f(nyw, kb), ay, byn);
-                float tnz = __builtin_fmaf(byte_f(nzw, kb), az, bzn);
-                float tfx = __builtin_fmaf(byte_f(fxw, kb), ax, bxf);
-                float tfy = __builtin_fmaf(byte_f(fyw, kb), ay, byf);
-                float tfz = __builtin_fmaf(byte_f(fzw, kb), az, bzf);
-                float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
-                float tf = fminf(fminf(fminf(tfx, tfy), tfz), tbest);
+                const float tnx = __builtin_fmaf(byte_f(nxw, kb), ax, bxn);
+                const float tny = __builtin_fmaf(byte_f(nyw, kb), ay, byn);
+                const float tnz = __builtin_fmaf(byte_f(nzw, kb), az, bzn);
+                const float tfx = __builtin_fmaf(byte_f(fxw, kb), ax, bxf);
+                const float tfy = __builtin_fmaf(byte_f(fyw, kb), ay, byf);
+                const float tfz = __builtin_fmaf(byte_f(fzw, kb), az, bzf);
+                const float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
+                const float tf = fminf(fminf(fminf(tfx, tfy), tfz), tbest);
                 tk[k] = tn;
                 const uint32_t mb = ((k < 4 ? meta0 : meta1) >> (8 * kb)) & 0xffu;
                 if (tn <= tf) {
@@ -205,14 +305,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
                 for (uint32_t j = 0; j < cnt; ++j) {
                     const uint32_t rec = first + j;
                     const float* tp = sc.tris + 9 * (size_t)rec;
-                    f3 p0 = mk3(tp[0], tp[1], tp[2]);
-                    f3 p1 = mk3(tp[3], tp[4], tp[5]);
-                    f3 p2 = mk3(tp[6], tp[7], tp[8]);
+                    const f3 p0 = mk3(tp[0], tp[1], tp[2]);
+                    const f3 p1 = mk3(tp[3], tp[4], tp[5]);
+                    const f3 p2 = mk3(tp[6], tp[7], tp[8]);
                     if (kCount) ++n_tris;
                     TriCandidate c;
                     if (tri_test(wr, o, p0, p1, p2, tmin, tbest, c)) {
                         if (c.t <= tbest) {
-                            int32_t prim = sc.prim_ids[rec];
+                            const int32_t prim = sc.prim_ids[rec];
                             if (c.t < tbest || best_prim < 0 || prim < best_prim) {
                                 tbest = c.t;
                                 best_prim = prim;
@@ -246,12 +346,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
                 }
                 if (sp - bottom == kStackLds) {
                     if (bottom >= spill_cap) { atomicOr(err, 1u); break; }
-                    int bslot = (bottom & (kStackLds - 1)) * kTraceBlock + tid;
+                    const int bslot = (bottom & (kStackLds - 1)) * kTraceBlock + tid;
                     my_spill[bottom] = make_uint2(st_node[bslot], __float_as_uint(st_t[bslot]));
                     ++bottom;
                     if (kCount) ++n_spill;
                 }
-                int slot = (sp & (kStackLds - 1)) * kTraceBlock + tid;
+                const int slot = (sp & (kStackLds - 1)) * kTraceBlock + tid;
                 st_node[slot] = (uint32_t)push_n;
                 st_t[slot] = push_t;
                 ++sp;
@@ -269,11 +369,36 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
             n_spill += __shfl_down(n_spill, off, 64);
         }
         if (lane == 0) {
-            atomicAdd(&tl.stats[0], n_nodes);
-            atomicAdd(&tl.stats[1], n_tris);
-            atomicAdd(&tl.stats[2], n_spill);
+            unsigned long long* st = kMode == kModeTrace ? tl.stats : pl.stats;
+            atomicAdd(&st[0], n_nodes);
+            atomicAdd(&st[1], n_tris);
+            atomicAdd(&st[2], n_spill);
         }
     }
+    if constexpr (kMode != kModeTrace) {
+        unsigned long long r = n_rays;
+        for (int off = 32; off > 0; off >>= 1) r += __shfl_down(r, off, 64);
+        if (lane == 0 && r) atomicAdd(&pl.stats[3], r);
+        if constexpr (kCount) {
+            __syncthreads();
+            if (tid < kMaxQueueDepth && depth_hist[tid])
+                atomicAdd(&pl.depth_rays[tid], (unsigned long long)depth_hist[tid]);
+        }
+    }
+}
+
+template <int kMode>
+hipError_t launch_mode(prt_hip_ctx* ctx, const TraceLaunch& t, const PathLaunch& p, bool instrumented)
+{
+    SceneDevice sc{ctx->d_nodes, ctx->d_tris, ctx->d_prim_ids, ctx->tri_count};
+    uint32_t* err = ctx->d_counters + 32;
+    if (instrumented)
+        k_trace<true, kMode><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(sc, t, p, ctx->d_spill,
+                                                                               ctx->stack_entries, err);
+    else
+        k_trace<false, kMode><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(sc, t, p, ctx->d_spill,
+                                                                                ctx->stack_entries, err);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -282,14 +407,15 @@ int ensure_trace_resources(prt_hip_ctx* ctx)
 {
     if (ctx->trace_grid == 0) {
         int per_cu = 0;
-        PRT_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false>, kTraceBlock, 0));
+        PRT_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, kModeDiffuse>,
+                                                                  kTraceBlock, 0));
         if (per_cu < 1) per_cu = 1;
         ctx->trace_grid = per_cu * ctx->num_cus;
     }
     // worst case stack: 7 pushes per level of the tree, plus slack
     int need = 7 * (ctx->bvh_depth + 1) + 8;
     if (need < kStackLds) need = kStackLds;
-    int64_t bytes = (int64_t)ctx->trace_grid * kTraceBlock * need * (int64_t)sizeof(uint2);
+    const int64_t bytes = (int64_t)ctx->trace_grid * kTraceBlock * need * (int64_t)sizeof(uint2);
     if (bytes > ctx->spill_bytes || need != ctx->stack_entries) {
         if (ctx->d_spill) { (void)hipFree(ctx->d_spill); ctx->d_spill = nullptr; }
         PRT_TRY(ctx, hipMalloc(&ctx->d_spill, bytes));
@@ -301,15 +427,14 @@ int ensure_trace_resources(prt_hip_ctx* ctx)
 
 hipError_t launch_trace(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumented)
 {
-    SceneDevice sc{ctx->d_nodes, ctx->d_tris, ctx->d_prim_ids, ctx->tri_count};
-    uint32_t* err = ctx->d_counters + 32;
-    if (instrumented)
-        k_trace<true><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(sc, t, ctx->d_spill,
-                                                                        ctx->stack_entries, err);
-    else
-        k_trace<false><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(sc, t, ctx->d_spill,
-                                                                         ctx->stack_entries, err);
-    return hipGetLastError();
+    return launch_mode<kModeTrace>(ctx, t, PathLaunch{}, instrumented);
+}
+
+hipError_t launch_paths(prt_hip_ctx* ctx, const PathLaunch& p, bool primary, bool instrumented)
+{
+    TraceLaunch t{};
+    return primary ? launch_mode<kModePrimary>(ctx, t, p, instrumented)
+                   : launch_mode<kModeDiffuse>(ctx, t, p, instrumented);
 }
 
 }  // namespace prt

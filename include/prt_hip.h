@@ -88,7 +88,14 @@ typedef struct prt_render_stats {
     double ms;
     double mray;
     int64_t paths;        /* live primary rays = W*H                       */
-    int64_t queue[16];    /* rays traced at depth k (k <= max_depth)       */
+    int64_t queue[16];    /* rays traced at depth k (k <= max_depth); with
+                             the fused pipeline only when PRT_PROFILE_COUNTERS
+                             is on, else 0                                  */
+    /* filled only when prt_hip_set_profiling enables them:                  */
+    double trace_ms;      /* summed device time of the closest-hit launches */
+    double shade_ms;      /* summed device time of the shading launches     */
+    int64_t nodes;        /* BVH8 nodes visited, all rays of the frame      */
+    int64_t tris;         /* triangle tests, all rays of the frame          */
 } prt_render_stats;
 
 /* Per-ray traversal counters (the PROFILE_MODE nodes/prims of
@@ -141,6 +148,12 @@ int prt_hip_intersect_counters(prt_hip_ctx* ctx, const prt_ray* rays, prt_hit* h
  * (camera/pinhole_camera.h:34-48).  Host-only; needs no context. */
 int prt_hip_camera_from_view(const float position[3], float angle_x, float angle_y, float fov_y,
                              float aspect_ratio, prt_camera* out_camera);
+/* The two halves of prt_hip_camera_from_view: the view basis U, V, N
+ * (9 floats) of RenderWindow::makeCamera, and the PinholeCamera of a
+ * position + basis (camera/pinhole_camera.h:34-48).  Host-only. */
+int prt_hip_view_basis(float angle_x, float angle_y, float out_basis[9]);
+int prt_hip_camera_from_basis(const float position[3], const float basis[9], float fov_y,
+                              float aspect_ratio, prt_camera* out_camera);
 int prt_hip_camera_set(prt_hip_ctx* ctx, const prt_camera* camera);
 int prt_hip_frame_init(prt_hip_ctx* ctx, int width, int height);
 int prt_hip_frame_clear(prt_hip_ctx* ctx);    /* also resets the pass counter */
@@ -153,6 +166,21 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo,
                    prt_render_stats* out_stats);
 /* Sets the pass index the next prt_hip_render call uses. */
 int prt_hip_set_pass(prt_hip_ctx* ctx, int pass);
+/* Profiling of prt_hip_render: PRT_PROFILE_TIMING records HIP events around
+ * every trace and shade launch (trace_ms, shade_ms); PRT_PROFILE_COUNTERS runs
+ * the instrumented traversal (nodes, tris).  0 turns both off. */
+#define PRT_PROFILE_TIMING   1
+#define PRT_PROFILE_COUNTERS 2
+int prt_hip_set_profiling(prt_hip_ctx* ctx, int flags);
+/* Frame pipeline of prt_hip_render.  PRT_PIPELINE_FUSED (default): one
+ * persistent launch per frame in which every lane generates, traces, shades
+ * and bounces whole paths.  PRT_PIPELINE_WAVEFRONT: the reference's structure
+ * (render/diffuse_renderer_cuda.cu:159-212), raygen then one closest-hit and
+ * one shading launch per depth with compacted ray queues in HBM (queue sizes
+ * stay on the device).  Both give bit-identical images. */
+#define PRT_PIPELINE_FUSED     0
+#define PRT_PIPELINE_WAVEFRONT 1
+int prt_hip_set_pipeline(prt_hip_ctx* ctx, int pipeline);
 
 /* --- frame buffer read-back -------------------------------------------- */
 /* Raw accumulation buffer, float4 (r, g, b, weight) per pixel, row-major. */

@@ -509,6 +509,64 @@ float refine_t(const Tri& tr, V3 o, V3 d, float t32)
     return (float)t;
 }
 
+// Ordered fp32 traversal for the CPU timing baseline: float slab tests with
+// the far bound widened by 2^-20 (robust against rounding), near child first,
+// culling by the current closest hit.  Same triangle test and (t, primId)
+// rule as closest_fp32; its answer can differ only for rays grazing a box
+// within float rounding.
+Closest32 closest_fp32_fast(const Scene& sc, V3 o, V3 d)
+{
+    Closest32 r;
+    r.t = 3.402823466e38f;
+    if (sc.nodes.empty() || sc.tris.empty()) return r;
+    WRay w = wsetup(d);
+    auto sinv = [](float x) { return 1.0f / (std::fabs(x) < 1e-20f ? (x < 0 ? -1e-20f : 1e-20f) : x); };
+    const float ix = sinv(d.x), iy = sinv(d.y), iz = sinv(d.z);
+    const float ox = o.x, oy = o.y, oz = o.z;
+    auto slab = [&](const Node& nd, float& tn) {
+        float ax = (nd.lo[0] - ox) * ix, bx = (nd.hi[0] - ox) * ix;
+        float ay = (nd.lo[1] - oy) * iy, by = (nd.hi[1] - oy) * iy;
+        float az = (nd.lo[2] - oz) * iz, bz = (nd.hi[2] - oz) * iz;
+        float t0 = std::max(std::max(std::min(ax, bx), std::min(ay, by)), std::max(std::min(az, bz), 0.0f));
+        float t1 = std::min(std::min(std::max(ax, bx), std::max(ay, by)), std::max(az, bz));
+        tn = t0;
+        return t0 <= t1 * 1.00000095367431641f && t0 <= r.t;
+    };
+    struct Ent { int node; float t; };
+    Ent stack[128];
+    int sp = 0;
+    float tn0;
+    if (!slab(sc.nodes[0], tn0)) return r;
+    stack[sp++] = {0, tn0};
+    while (sp) {
+        Ent e = stack[--sp];
+        if (e.t > r.t) continue;
+        const Node& nd = sc.nodes[e.node];
+        if (nd.left < 0) {
+            for (int i = nd.first; i < nd.first + nd.count; ++i) {
+                int id = sc.order[i];
+                float t, vn, wn, det;
+                if (!tri_test_f(w, o, sc.tris[id], 0.0f, r.t, t, vn, wn, det)) continue;
+                if (t < r.t || r.prim < 0 || (t == r.t && id < r.prim)) {
+                    r.t = t; r.prim = id; r.u = vn / det; r.v = wn / det;
+                }
+            }
+            continue;
+        }
+        float ta, tb;
+        bool ha = slab(sc.nodes[nd.left], ta), hb = slab(sc.nodes[nd.left + 1], tb);
+        if (ha && hb) {
+            if (ta <= tb) { stack[sp++] = {nd.left + 1, tb}; stack[sp++] = {nd.left, ta}; }
+            else { stack[sp++] = {nd.left, ta}; stack[sp++] = {nd.left + 1, tb}; }
+        } else if (ha) {
+            stack[sp++] = {nd.left, ta};
+        } else if (hb) {
+            stack[sp++] = {nd.left + 1, tb};
+        }
+    }
+    return r;
+}
+
 // ----------------------------------------------------------------- camera
 struct Cam {
     V3 origin, image_o, image_dx, image_dy;
@@ -897,7 +955,7 @@ int orc_stream_bench(void* scp, const float* cam, int w, int h, int row0, int ro
             float lw = 1.0f;
             for (int depth = 0;; ++depth) {
                 for (RayS& r : in) {
-                    Closest32 hc = closest_fp32(sc, r.o, r.d, 0.0f, 3.402823466e38f);
+                    Closest32 hc = closest_fp32_fast(sc, r.o, r.d);
                     r.prim = hc.prim;
                     r.t = hc.prim >= 0 ? refine_t(sc.tris[hc.prim], r.o, r.d, hc.t) : hc.t;
                 }

@@ -185,7 +185,14 @@ def test_intersect_tie_rule(dev):
     assert hits[0, 0] == np.float32(1.0)
 
 
-def test_render_primary_bit_exact(dev, orc, scenes):
+@pytest.fixture(params=["fused", "wavefront"])
+def pipeline(request, dev):
+    dev.set_pipeline(request.param)
+    yield request.param
+    dev.set_pipeline("fused")
+
+
+def test_render_primary_bit_exact(dev, orc, scenes, pipeline):
     mesh = scenes.scene_heightfield_box(1001)
     _load(dev, mesh)
     w, h = 256, 192
@@ -199,7 +206,7 @@ def test_render_primary_bit_exact(dev, orc, scenes):
     assert np.array_equal(dev.accum(), ref["accum"])
 
 
-def test_render_diffuse_bit_exact(dev, orc, scenes):
+def test_render_diffuse_bit_exact(dev, orc, scenes, pipeline):
     mesh = scenes.scene_heightfield_box(1001)
     _load(dev, mesh)
     w, h = 160, 128
@@ -246,6 +253,34 @@ def test_render_diffuse_3sigma_vs_reference_numerics(dev, orc, scenes):
     assert outside.mean() < 0.01, f"{outside.mean():.4f} of pixels outside 3 sigma"
     z = (mg.sum() - mr.sum()) / math.sqrt(((vg + vr) / spp).sum())
     assert abs(z) < 5.0, f"image means differ: z = {z:.2f}"
+
+
+def test_pipelines_agree_on_spheres_partial_tiles(dev, orc, scenes):
+    """Fused and wavefront frames are bit-identical to each other and to the
+    kernel-numerics oracle, with the same ray count per depth, on the
+    1.3M-triangle sphere scene at a size with partial 8x8 tiles."""
+    mesh = scenes.build_scene("spheres_1080p")
+    _load(dev, mesh)
+    w, h = 300, 170
+    cam = _cam(orc, scenes, "spheres_1080p", w, h)
+    dev.set_camera(cam)
+    dev.init_frame(w, h)
+    out = {}
+    for mode in ("fused", "wavefront"):
+        dev.set_pipeline(mode)
+        dev.set_profiling(counters=True)
+        dev.clear_frame()
+        st = [dev.render("diffuseHip", max_depth=8, albedo=0.7) for _ in range(2)]
+        out[mode] = (dev.accum(), [s["queue"] for s in st], [s["ray"] for s in st])
+    dev.set_profiling()
+    dev.set_pipeline("fused")
+    assert np.array_equal(out["fused"][0], out["wavefront"][0])
+    assert out["fused"][1] == out["wavefront"][1]
+    assert out["fused"][2] == out["wavefront"][2]
+    assert out["fused"][2][0] == sum(out["fused"][1][0])
+    ref = orc.Scene(mesh.idx, mesh.pos).render(cam, w, h, spp=2, max_depth=8, albedo=0.7, renderer=1, numerics=0)
+    assert np.array_equal(out["fused"][0], ref["accum"])
+    assert sum(out["fused"][2]) == ref["rays"]
 
 
 def test_known_answers(dev, orc, scenes):
