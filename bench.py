@@ -71,6 +71,64 @@ def cpu_baseline(scenes, mesh, cfg, cam, frames: int) -> dict:
                       f"{r['rays']} rays in {r['seconds']:.2f} s"}
 
 
+def bench_batched(pkg, scenes, dev, mesh, cam, cfg, args, plain_mrays: float, wavefront_mrays: float) -> dict:
+    """Batched form.  (1) spheres_1080p as one job of spp = steps: the same
+    frames as the plain run (passes warmup..warmup+steps-1) in one persistent
+    launch per render; (2) the 32-scene workload (triangles log-uniform
+    1k-2M, resolutions {256^2, 640x480, 1280x720, 1920x1080}, spp 1..16,
+    depth 6) in one launch, launch shape from the autotuner."""
+    w, h, depth, albedo = cfg["width"], cfg["height"], cfg["max_depth"], cfg["albedo"]
+    dev.batch_reset()
+    sid, _ = dev.batch_add_scene(mesh.idx, mesh.pos)
+    job = {"scene": sid, "width": w, "height": h, "spp": args.steps, "max_depth": depth,
+           "pass0": args.warmup, "albedo": albedo, "camera": cam}
+    dev.batch_set_jobs([job])
+    tune2 = dev.batch_autotune()
+    chosen2 = dev.batch_get_tuning()
+    dev.batch_set_jobs([job])
+    dev.batch_render()  # warm-up (also captures the graph)
+    reps = 3
+    t0 = time.perf_counter()
+    st = [dev.batch_render() for _ in range(reps)]
+    wall = time.perf_counter() - t0
+    rays = sum(x["ray"] for x in st)
+    c2 = rays / wall / 1e6
+    log(f"batched spheres_1080p ({args.steps} spp per launch): {c2:.1f} Mrays/s "
+        f"({c2 / plain_mrays:.2f}x plain fused, {c2 / wavefront_mrays:.2f}x plain wavefront), tuning {chosen2}")
+
+    dev.batch_reset()
+    specs = scenes.batch_workload(args.batch_scenes)
+    tris = 0
+    bjobs = []
+    for sp in specs:
+        m = scenes.scene_mix(sp["seed"], sp["target_tris"])
+        tris += m.tri_count
+        sid, _ = dev.batch_add_scene(m.idx, m.pos)
+        v = sp["view"]
+        bcam = pkg.camera_from_view(v["pos"], v["angle_x"], v["angle_y"], v["fov_y"], sp["width"] / sp["height"])
+        bjobs.append({"scene": sid, "width": sp["width"], "height": sp["height"], "spp": sp["spp"],
+                      "max_depth": sp["max_depth"], "albedo": sp["albedo"], "camera": bcam})
+    dev.batch_set_jobs(bjobs)
+    tune3 = dev.batch_autotune()
+    chosen3 = dev.batch_get_tuning()
+    dev.batch_render()
+    t0 = time.perf_counter()
+    st3 = [dev.batch_render() for _ in range(2)]
+    wall3 = time.perf_counter() - t0
+    rays3 = sum(x["ray"] for x in st3)
+    c3 = rays3 / wall3 / 1e6
+    log(f"batched {len(specs)} scenes ({tris} triangles, {st3[0]['paths']} paths per launch): {c3:.1f} Mrays/s, "
+        f"tuning {chosen3}")
+    dev.batch_reset()
+    return {"spheres_1080p_mrays": round(c2, 2), "spheres_1080p_spp_per_launch": args.steps,
+            "ratio_vs_plain_fused": round(c2 / plain_mrays, 3),
+            "ratio_vs_plain_wavefront": round(c2 / wavefront_mrays, 3),
+            "spheres_1080p_tuning": chosen2,
+            "scenes32_mrays": round(c3, 2), "scenes32_triangles": tris, "scenes32_paths": st3[0]["paths"],
+            "scenes32_rays_per_launch": st3[0]["ray"], "scenes32_tuning": chosen3,
+            "autotune_candidates": len(tune3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +136,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-frames", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-scenes", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is "
@@ -150,6 +210,11 @@ def main() -> int:
     assert wrays == rays, "wavefront and fused pipelines traced different ray counts"
     log(f"wavefront pipeline: {wrays / wwall / 1e6:.1f} Mrays/s")
 
+    # batched form, config 2: the 16 spp of the same frame sequence in one launch
+    batched = None
+    if not args.no_batched:
+        batched = bench_batched(pkg, scenes, dev, mesh, cam, cfg, args, value, wrays / wwall / 1e6)
+
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
@@ -170,6 +235,7 @@ def main() -> int:
         "nodes_per_ray": round(n_node, 3), "tris_per_ray": round(n_tri, 3),
         "rays_per_frame": rays // args.steps, "rays_per_depth": depth_rays,
         "build": {"buildMs": round(build["buildMs"], 3), "buildMprim": round(build["buildMprim"], 2)},
+        "batched": batched,
         "cpu_baseline": None,
     }
     dev.close()

@@ -53,6 +53,25 @@ class PrtTraceCounters(ctypes.Structure):
                 ("spills", ctypes.c_int64)]
 
 
+class PrtBatchJob(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_int32), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("spp", ctypes.c_int32), ("max_depth", ctypes.c_int32), ("pass0", ctypes.c_int32),
+                ("albedo", ctypes.c_float), ("camera", PrtCamera)]
+
+
+class PrtBatchStats(ctypes.Structure):
+    _fields_ = [("rays", ctypes.c_int64), ("paths", ctypes.c_int64), ("ms", ctypes.c_double),
+                ("mray", ctypes.c_double), ("nodes", ctypes.c_int64), ("tris", ctypes.c_int64)]
+
+
+class PrtBatchTuning(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int32), ("block", ctypes.c_int32), ("stack", ctypes.c_int32),
+                ("blocks_per_cu", ctypes.c_int32), ("refill", ctypes.c_int32), ("ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def declared_symbols(header: str = HEADER_PATH) -> list[str]:
     """Every function name declared in include/prt_hip.h."""
     text = open(header).read()
@@ -99,6 +118,29 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "prt_hip_bvh_info": ([P, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_int32)],
                              ctypes.c_int),
         "prt_hip_bvh_read": ([P, P, P, P], ctypes.c_int),
+        "prt_hip_set_albedo_host": ([P, P], ctypes.c_int),
+        "prt_hip_set_albedo_device": ([P, P], ctypes.c_int),
+        "prt_hip_render_albedo_image": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P,
+                                         ctypes.POINTER(PrtRenderStats)], ctypes.c_int),
+        "prt_hip_render_albedo_image_host": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P,
+                                              ctypes.POINTER(PrtRenderStats)], ctypes.c_int),
+        "prt_hip_adjoint_albedo": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P,
+                                    ctypes.POINTER(PrtRenderStats)], ctypes.c_int),
+        "prt_hip_adjoint_albedo_host": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P,
+                                         ctypes.POINTER(PrtRenderStats)], ctypes.c_int),
+        "prt_hip_view_basis": ([ctypes.c_float, ctypes.c_float, P], ctypes.c_int),
+        "prt_hip_camera_from_basis": ([P, P, ctypes.c_float, ctypes.c_float, ctypes.POINTER(PrtCamera)],
+                                      ctypes.c_int),
+        "prt_hip_batch_scene_add": ([P, P, i64, P, i64, ctypes.POINTER(PrtBuildStats),
+                                     ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+        "prt_hip_batch_reset": ([P], ctypes.c_int),
+        "prt_hip_batch_set_jobs": ([P, P, ctypes.c_int], ctypes.c_int),
+        "prt_hip_batch_render": ([P, ctypes.POINTER(PrtBatchStats)], ctypes.c_int),
+        "prt_hip_batch_accum_read": ([P, ctypes.c_int, P], ctypes.c_int),
+        "prt_hip_batch_set_tuning": ([P, ctypes.POINTER(PrtBatchTuning)], ctypes.c_int),
+        "prt_hip_batch_get_tuning": ([P, ctypes.POINTER(PrtBatchTuning)], ctypes.c_int),
+        "prt_hip_batch_autotune": ([P, P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "prt_hip_batch_set_graph": ([P, ctypes.c_int], ctypes.c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)
@@ -265,6 +307,105 @@ class HipDevice:
         out = np.zeros((self.height, self.width), dtype=np.int32)
         self._check(self.lib.prt_hip_frame_read_bgr8(self.ctx, _ptr(out)), "frame_read_bgr8")
         return out
+
+    # -- per-triangle RGB albedo and its adjoint
+    def set_albedo(self, rgb: np.ndarray) -> None:
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32).reshape(-1, 3)
+        if rgb.shape[0] != self.tri_count:
+            raise ValueError("albedo needs 3 floats per triangle")
+        self._check(self.lib.prt_hip_set_albedo_host(self.ctx, _ptr(rgb)), "set_albedo")
+
+    def set_albedo_device(self, ptr: int) -> None:
+        self._check(self.lib.prt_hip_set_albedo_device(self.ctx, ctypes.c_void_p(ptr)), "set_albedo_device")
+
+    @staticmethod
+    def _stats(st: PrtRenderStats) -> dict:
+        return {"ray": st.rays, "paths": st.paths, "renderMs": st.ms, "mray": st.mray, "nodes": st.nodes,
+                "tris": st.tris}
+
+    def render_albedo_image(self, pass0: int, spp: int, max_depth: int) -> tuple[np.ndarray, dict]:
+        img = np.zeros((self.height, self.width, 3), dtype=np.float32)
+        st = PrtRenderStats()
+        self._check(self.lib.prt_hip_render_albedo_image_host(self.ctx, pass0, spp, max_depth, _ptr(img),
+                                                              ctypes.byref(st)), "render_albedo_image")
+        return img, self._stats(st)
+
+    def render_albedo_image_device(self, pass0: int, spp: int, max_depth: int, image_ptr: int) -> dict:
+        st = PrtRenderStats()
+        self._check(self.lib.prt_hip_render_albedo_image(self.ctx, pass0, spp, max_depth, ctypes.c_void_p(image_ptr),
+                                                         ctypes.byref(st)), "render_albedo_image")
+        return self._stats(st)
+
+    def adjoint_albedo(self, pass0: int, spp: int, max_depth: int, dldi: np.ndarray) -> tuple[np.ndarray, dict]:
+        dldi = np.ascontiguousarray(dldi, dtype=np.float32).reshape(self.height, self.width, 3)
+        grad = np.zeros((self.tri_count, 3), dtype=np.float32)
+        st = PrtRenderStats()
+        self._check(self.lib.prt_hip_adjoint_albedo_host(self.ctx, pass0, spp, max_depth, _ptr(dldi), _ptr(grad),
+                                                         ctypes.byref(st)), "adjoint_albedo")
+        return grad, self._stats(st)
+
+    def adjoint_albedo_device(self, pass0: int, spp: int, max_depth: int, dldi_ptr: int, grad_ptr: int) -> dict:
+        st = PrtRenderStats()
+        self._check(self.lib.prt_hip_adjoint_albedo(self.ctx, pass0, spp, max_depth, ctypes.c_void_p(dldi_ptr),
+                                                    ctypes.c_void_p(grad_ptr), ctypes.byref(st)),
+                    "adjoint_albedo")
+        return self._stats(st)
+
+    # -- batched form
+    def batch_add_scene(self, idx: np.ndarray, pos: np.ndarray) -> tuple[int, dict]:
+        idx = np.ascontiguousarray(idx, dtype=np.int32).reshape(-1, 3)
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        st = PrtBuildStats()
+        sid = ctypes.c_int32()
+        self._check(self.lib.prt_hip_batch_scene_add(self.ctx, _ptr(idx), idx.shape[0], _ptr(pos), pos.shape[0],
+                                                     ctypes.byref(st), ctypes.byref(sid)), "batch_scene_add")
+        return sid.value, {"buildMs": st.build_ms, "buildMprim": st.build_mprim, "nodes": st.node_count,
+                           "depth": st.depth}
+
+    def batch_reset(self) -> None:
+        self._check(self.lib.prt_hip_batch_reset(self.ctx), "batch_reset")
+
+    def batch_set_jobs(self, jobs: list[dict]) -> None:
+        arr = (PrtBatchJob * len(jobs))()
+        for a, j in zip(arr, jobs):
+            a.scene, a.width, a.height = j["scene"], j["width"], j["height"]
+            a.spp, a.max_depth, a.pass0 = j["spp"], j["max_depth"], j.get("pass0", 0)
+            a.albedo = j.get("albedo", 0.8)
+            a.camera = camera_struct(j["camera"])
+        self._batch_sizes = [(j["height"], j["width"]) for j in jobs]
+        self._check(self.lib.prt_hip_batch_set_jobs(self.ctx, ctypes.cast(arr, ctypes.c_void_p), len(jobs)),
+                    "batch_set_jobs")
+
+    def batch_render(self) -> dict:
+        st = PrtBatchStats()
+        self._check(self.lib.prt_hip_batch_render(self.ctx, ctypes.byref(st)), "batch_render")
+        return {"ray": st.rays, "paths": st.paths, "renderMs": st.ms, "mray": st.mray, "nodes": st.nodes,
+                "tris": st.tris}
+
+    def batch_accum(self, job: int) -> np.ndarray:
+        h, w = self._batch_sizes[job]
+        out = np.zeros((h, w, 4), dtype=np.float32)
+        self._check(self.lib.prt_hip_batch_accum_read(self.ctx, job, _ptr(out)), "batch_accum_read")
+        return out
+
+    def batch_get_tuning(self) -> dict:
+        t = PrtBatchTuning()
+        self._check(self.lib.prt_hip_batch_get_tuning(self.ctx, ctypes.byref(t)), "batch_get_tuning")
+        return t.as_dict()
+
+    def batch_set_tuning(self, variant: int, blocks_per_cu: int = 0, refill: int = 24) -> None:
+        t = PrtBatchTuning(variant=variant, blocks_per_cu=blocks_per_cu, refill=refill)
+        self._check(self.lib.prt_hip_batch_set_tuning(self.ctx, ctypes.byref(t)), "batch_set_tuning")
+
+    def batch_autotune(self) -> list[dict]:
+        res = (PrtBatchTuning * 64)()
+        n = ctypes.c_int()
+        self._check(self.lib.prt_hip_batch_autotune(self.ctx, ctypes.cast(res, ctypes.c_void_p), 64,
+                                                    ctypes.byref(n)), "batch_autotune")
+        return [res[i].as_dict() for i in range(n.value)]
+
+    def batch_set_graph(self, enable: bool) -> None:
+        self._check(self.lib.prt_hip_batch_set_graph(self.ctx, 1 if enable else 0), "batch_set_graph")
 
     def debug_raygen(self, pass_index: int) -> tuple[np.ndarray, np.ndarray]:
         rays = np.zeros((self.height * self.width, 8), dtype=np.float32)

@@ -16,6 +16,21 @@
 
 namespace prt {
 
+void SceneBuffers::release()
+{
+    void* ptrs[] = {d_idx, d_pos, d_nodes, d_tris, d_prim_ids};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    d_idx = nullptr;
+    d_pos = nullptr;
+    d_nodes = nullptr;
+    d_tris = nullptr;
+    d_prim_ids = nullptr;
+    bvh_ready = false;
+    node_count = 0;
+    bvh_depth = 0;
+}
+
 int set_error(prt_hip_ctx* ctx, const std::string& msg)
 {
     if (ctx) ctx->err = msg;
@@ -101,7 +116,7 @@ int ensure_io(prt_hip_ctx* ctx, int64_t n)
 int intersect_impl(prt_hip_ctx* ctx, const prt_ray* d_rays, prt_hit* d_hits, int64_t n,
                    unsigned long long* d_stats)
 {
-    if (!ctx->bvh_ready) return set_error(ctx, "intersect: no BVH (call prt_hip_bvh_build)");
+    if (!ctx->scene.bvh_ready) return set_error(ctx, "intersect: no BVH (call prt_hip_bvh_build)");
     if (n < 0 || n > (int64_t)UINT32_MAX / 2) return set_error(ctx, "intersect: bad ray count");
     if (ensure_trace_resources(ctx)) return 1;
     PRT_TRY(ctx, launch_zero_u32(ctx->d_counters, 33, ctx->stream));
@@ -137,7 +152,7 @@ int prt_hip_create(int device_index, prt_hip_ctx** out_ctx)
         hipGetDeviceProperties(&prop, device_index) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
-        hipMalloc(&ctx->d_counters, 64 * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&ctx->d_counters, 64 * sizeof(uint32_t)) != hipSuccess ||  // [48] batch work counter
         hipMalloc(&ctx->d_stats, 32 * sizeof(unsigned long long)) != hipSuccess) {
         prt_hip_destroy(ctx);
         return 5;
@@ -158,7 +173,10 @@ int prt_hip_destroy(prt_hip_ctx* ctx)
     if (!ctx) return 0;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    void* ptrs[] = {ctx->d_idx, ctx->d_pos, ctx->d_nodes, ctx->d_tris, ctx->d_prim_ids, ctx->d_spill,
+    ctx->scene.release();
+    prt_hip_batch_reset(ctx);
+    free_ptr(ctx->d_albedo_own);
+    void* ptrs[] = {ctx->d_spill,
                     ctx->d_accum, ctx->d_bgr8, ctx->d_rays[0], ctx->d_rays[1], ctx->d_hits,
                     ctx->d_pix[0], ctx->d_pix[1], ctx->d_rng[0], ctx->d_rng[1], ctx->d_counters,
                     ctx->d_stats, ctx->d_io_rays, ctx->d_io_hits};
@@ -193,18 +211,21 @@ int prt_hip_scene_upload(prt_hip_ctx* ctx, const int32_t* idx3, int64_t tri_coun
     for (int64_t i = 0; i < 3 * tri_count; ++i)
         if (idx3[i] < 0 || idx3[i] >= vertex_count)
             return set_error(ctx, "scene_upload: vertex index out of range");
-    free_ptr(ctx->d_idx);
-    free_ptr(ctx->d_pos);
-    ctx->d_idx = nullptr;
-    ctx->d_pos = nullptr;
-    ctx->bvh_ready = false;
-    ctx->tri_count = tri_count;
-    ctx->vertex_count = vertex_count;
-    PRT_TRY(ctx, hipMalloc(&ctx->d_idx, std::max<int64_t>(16, 12 * tri_count)));
-    PRT_TRY(ctx, hipMalloc(&ctx->d_pos, std::max<int64_t>(16, 12 * vertex_count)));
+    free_ptr(ctx->scene.d_idx);
+    free_ptr(ctx->scene.d_pos);
+    ctx->scene.d_idx = nullptr;
+    ctx->scene.d_pos = nullptr;
+    ctx->scene.bvh_ready = false;
+    ctx->scene.tri_count = tri_count;
+    ctx->scene.vertex_count = vertex_count;
+    free_ptr(ctx->d_albedo_own);
+    ctx->d_albedo_own = nullptr;
+    ctx->d_albedo = nullptr;
+    PRT_TRY(ctx, hipMalloc(&ctx->scene.d_idx, std::max<int64_t>(16, 12 * tri_count)));
+    PRT_TRY(ctx, hipMalloc(&ctx->scene.d_pos, std::max<int64_t>(16, 12 * vertex_count)));
     if (tri_count) {
-        PRT_TRY(ctx, hipMemcpyAsync(ctx->d_idx, idx3, 12 * tri_count, hipMemcpyHostToDevice, ctx->stream));
-        PRT_TRY(ctx, hipMemcpyAsync(ctx->d_pos, pos3, 12 * vertex_count, hipMemcpyHostToDevice, ctx->stream));
+        PRT_TRY(ctx, hipMemcpyAsync(ctx->scene.d_idx, idx3, 12 * tri_count, hipMemcpyHostToDevice, ctx->stream));
+        PRT_TRY(ctx, hipMemcpyAsync(ctx->scene.d_pos, pos3, 12 * vertex_count, hipMemcpyHostToDevice, ctx->stream));
     }
     PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
@@ -213,8 +234,8 @@ int prt_hip_scene_upload(prt_hip_ctx* ctx, const int32_t* idx3, int64_t tri_coun
 int prt_hip_bvh_build(prt_hip_ctx* ctx, prt_build_stats* out_stats)
 {
     if (use_device(ctx)) return 1;
-    if (!ctx->d_idx) return set_error(ctx, "bvh_build: no scene uploaded");
-    if (build_bvh8(ctx, out_stats)) return 1;
+    if (!ctx->scene.d_idx) return set_error(ctx, "bvh_build: no scene uploaded");
+    if (build_bvh8(ctx, ctx->scene, out_stats)) return 1;
     return ensure_trace_resources(ctx);
 }
 
@@ -386,7 +407,7 @@ int prt_hip_set_pipeline(prt_hip_ctx* ctx, int pipeline)
 int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, prt_render_stats* out)
 {
     if (use_device(ctx)) return 1;
-    if (!ctx->bvh_ready) return set_error(ctx, "render: no BVH");
+    if (!ctx->scene.bvh_ready) return set_error(ctx, "render: no BVH");
     if (!ctx->cam_ready) return set_error(ctx, "render: no camera");
     if (!ctx->d_accum) return set_error(ctx, "render: no frame");
     if (renderer != PRT_RENDERER_PRIMARY && renderer != PRT_RENDERER_DIFFUSE)
@@ -493,6 +514,493 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
     return 0;
 }
 
+namespace {
+
+void batch_drop_graph(prt_hip_ctx* ctx)
+{
+    if (ctx->batch_graph_exec) (void)hipGraphExecDestroy(ctx->batch_graph_exec);
+    if (ctx->batch_graph) (void)hipGraphDestroy(ctx->batch_graph);
+    ctx->batch_graph_exec = nullptr;
+    ctx->batch_graph = nullptr;
+}
+
+void batch_free_jobs(prt_hip_ctx* ctx)
+{
+    batch_drop_graph(ctx);
+    void* ptrs[] = {ctx->d_batch_jobs, ctx->d_batch_work_begin, ctx->d_batch_pixel_begin, ctx->d_batch_samples,
+                    ctx->d_batch_accum};
+    for (void* p : ptrs) free_ptr(p);
+    ctx->d_batch_jobs = nullptr;
+    ctx->d_batch_work_begin = nullptr;
+    ctx->d_batch_pixel_begin = nullptr;
+    ctx->d_batch_samples = nullptr;
+    ctx->d_batch_accum = nullptr;
+    ctx->batch_jobs.clear();
+    ctx->batch_samples_len = ctx->batch_pixels = 0;
+    ctx->batch_total_work = 0;
+}
+
+BatchLaunch batch_launch_args(prt_hip_ctx* ctx)
+{
+    BatchLaunch bl{};
+    bl.jobs = ctx->d_batch_jobs;
+    bl.work_begin = ctx->d_batch_work_begin;
+    bl.n_jobs = (int)ctx->batch_jobs.size();
+    bl.scenes = ctx->d_batch_scenes;
+    bl.samples = ctx->d_batch_samples;
+    bl.total_work = ctx->batch_total_work;
+    bl.work_counter = ctx->d_counters + 48;
+    bl.stats = ctx->d_stats;
+    return bl;
+}
+
+int batch_resolve_tuning(prt_hip_ctx* ctx, BatchTuning& tu)
+{
+    if (tu.variant < 0 || tu.variant >= kBatchVariantCount) return set_error(ctx, "batch: bad kernel variant");
+    int max_per_cu = 0;
+    PRT_TRY(ctx, launch_batch(ctx, BatchLaunch{}, tu, false, &max_per_cu));
+    const int lane_cap = 2048 / kBatchVariants[tu.variant].block;
+    if (max_per_cu > lane_cap) max_per_cu = lane_cap;
+    if (max_per_cu < 1) max_per_cu = 1;
+    if (tu.blocks_per_cu <= 0 || tu.blocks_per_cu > max_per_cu) tu.blocks_per_cu = max_per_cu;
+    if (tu.refill < 1) tu.refill = 1;
+    if (tu.refill > 64) tu.refill = 64;
+    return 0;
+}
+
+int batch_enqueue(prt_hip_ctx* ctx, const BatchTuning& tu, bool instrumented, bool full)
+{
+    PRT_TRY(ctx, launch_batch_begin(ctx));
+    PRT_TRY(ctx, launch_batch(ctx, batch_launch_args(ctx), tu, instrumented, nullptr));
+    if (full) {
+        PRT_TRY(ctx, launch_batch_finalize(ctx));
+        PRT_TRY(ctx, launch_batch_advance(ctx));
+    }
+    return 0;
+}
+
+}  // namespace
+
+int prt_hip_batch_scene_add(prt_hip_ctx* ctx, const int32_t* idx3, int64_t tri_count, const float* pos3,
+                            int64_t vertex_count, prt_build_stats* out_stats, int32_t* out_scene)
+{
+    if (use_device(ctx)) return 1;
+    if (tri_count <= 0 || vertex_count <= 0 || !idx3 || !pos3 || !out_scene)
+        return set_error(ctx, "batch_scene_add: bad arguments");
+    if (tri_count >= ((int64_t)1 << 31) || vertex_count >= ((int64_t)1 << 31))
+        return set_error(ctx, "batch_scene_add: more than 2^31-1 triangles or vertices");
+    for (int64_t i = 0; i < 3 * tri_count; ++i)
+        if (idx3[i] < 0 || idx3[i] >= vertex_count) return set_error(ctx, "batch_scene_add: vertex index out of range");
+    SceneBuffers sb;
+    sb.tri_count = tri_count;
+    sb.vertex_count = vertex_count;
+    hipError_t e = hipMalloc(&sb.d_idx, 12 * tri_count);
+    if (e == hipSuccess) e = hipMalloc(&sb.d_pos, 12 * vertex_count);
+    if (e == hipSuccess) e = hipMemcpyAsync(sb.d_idx, idx3, 12 * tri_count, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(sb.d_pos, pos3, 12 * vertex_count, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        sb.release();
+        return check_hip(ctx, e, "batch_scene_add");
+    }
+    if (build_bvh8(ctx, sb, out_stats)) {
+        sb.release();
+        return 1;
+    }
+    batch_free_jobs(ctx);  // jobs refer to the scene table, which is rebuilt
+    ctx->batch_scenes.push_back(sb);
+    free_ptr(ctx->d_batch_scenes);
+    ctx->d_batch_scenes = nullptr;
+    std::vector<SceneDevice> table;
+    for (const SceneBuffers& b : ctx->batch_scenes) table.push_back(b.device());
+    PRT_TRY(ctx, hipMalloc(&ctx->d_batch_scenes, sizeof(SceneDevice) * table.size()));
+    PRT_TRY(ctx, hipMemcpy(ctx->d_batch_scenes, table.data(), sizeof(SceneDevice) * table.size(),
+                           hipMemcpyHostToDevice));
+    *out_scene = (int32_t)ctx->batch_scenes.size() - 1;
+    return ensure_trace_resources(ctx);
+}
+
+int prt_hip_batch_reset(prt_hip_ctx* ctx)
+{
+    if (!ctx) return 1;
+    (void)hipSetDevice(ctx->device);
+    batch_free_jobs(ctx);
+    for (SceneBuffers& b : ctx->batch_scenes) b.release();
+    ctx->batch_scenes.clear();
+    free_ptr(ctx->d_batch_scenes);
+    ctx->d_batch_scenes = nullptr;
+    return 0;
+}
+
+int prt_hip_batch_set_jobs(prt_hip_ctx* ctx, const prt_batch_job* jobs, int n_jobs)
+{
+    if (use_device(ctx)) return 1;
+    if (!jobs || n_jobs <= 0) return set_error(ctx, "batch_set_jobs: no jobs");
+    std::vector<BatchJobDevice> dj(n_jobs);
+    std::vector<uint32_t> work_begin(n_jobs + 1);
+    std::vector<uint64_t> pixel_begin(n_jobs + 1);
+    uint64_t work = 0, samples = 0, pixels = 0;
+    for (int j = 0; j < n_jobs; ++j) {
+        const prt_batch_job& in = jobs[j];
+        if (in.scene < 0 || in.scene >= (int)ctx->batch_scenes.size())
+            return set_error(ctx, "batch_set_jobs: unknown scene");
+        if (in.width <= 0 || in.height <= 0 || (int64_t)in.width * in.height > ((int64_t)1 << 28))
+            return set_error(ctx, "batch_set_jobs: bad image size");
+        if (in.spp < 1 || in.max_depth < 0 || in.max_depth + 1 >= kMaxQueueDepth)
+            return set_error(ctx, "batch_set_jobs: spp must be >= 1 and max_depth in [0, 14]");
+        if (in.camera.lens_radius != 0.0f) return set_error(ctx, "batch_set_jobs: only the pinhole camera");
+        BatchJobDevice& d = dj[j];
+        d.cam = to_camera(in.camera);
+        d.scene = in.scene;
+        d.width = in.width;
+        d.height = in.height;
+        d.tiles_x = (in.width + 7) / 8;
+        d.entries = (uint32_t)((int64_t)d.tiles_x * ((in.height + 7) / 8) * 64);
+        d.spp = in.spp;
+        d.pass0 = in.pass0;
+        d.max_depth = in.max_depth;
+        d.albedo = in.albedo;
+        d.inv_w = 1.0f / (float)in.width;
+        d.inv_h = 1.0f / (float)in.height;
+        d.work_begin = (uint32_t)work;
+        d.sample_base = samples;
+        d.pixel_base = pixels;
+        work_begin[j] = (uint32_t)work;
+        pixel_begin[j] = pixels;
+        work += (uint64_t)d.entries * (uint64_t)in.spp;
+        samples += (uint64_t)in.spp * (uint64_t)in.width * (uint64_t)in.height;
+        pixels += (uint64_t)in.width * (uint64_t)in.height;
+        if (work >= (uint64_t)UINT32_MAX - (1u << 20))
+            return set_error(ctx, "batch_set_jobs: more than 2^32 work items; split the batch");
+    }
+    work_begin[n_jobs] = (uint32_t)work;
+    pixel_begin[n_jobs] = pixels;
+    batch_free_jobs(ctx);
+    PRT_TRY(ctx, hipMalloc(&ctx->d_batch_jobs, sizeof(BatchJobDevice) * n_jobs));
+    PRT_TRY(ctx, hipMalloc(&ctx->d_batch_work_begin, sizeof(uint32_t) * (n_jobs + 1)));
+    PRT_TRY(ctx, hipMalloc(&ctx->d_batch_pixel_begin, sizeof(uint64_t) * (n_jobs + 1)));
+    PRT_TRY(ctx, hipMalloc(&ctx->d_batch_samples, sizeof(float) * samples));
+    PRT_TRY(ctx, hipMalloc(&ctx->d_batch_accum, sizeof(float4) * pixels));
+    PRT_TRY(ctx, hipMemcpy(ctx->d_batch_jobs, dj.data(), sizeof(BatchJobDevice) * n_jobs, hipMemcpyHostToDevice));
+    PRT_TRY(ctx, hipMemcpy(ctx->d_batch_work_begin, work_begin.data(), sizeof(uint32_t) * (n_jobs + 1),
+                           hipMemcpyHostToDevice));
+    PRT_TRY(ctx, hipMemcpy(ctx->d_batch_pixel_begin, pixel_begin.data(), sizeof(uint64_t) * (n_jobs + 1),
+                           hipMemcpyHostToDevice));
+    PRT_TRY(ctx, hipMemset(ctx->d_batch_accum, 0, sizeof(float4) * pixels));
+    ctx->batch_jobs = dj;
+    ctx->batch_samples_len = samples;
+    ctx->batch_pixels = pixels;
+    ctx->batch_total_work = (uint32_t)work;
+    return batch_resolve_tuning(ctx, ctx->batch_tuning);
+}
+
+int prt_hip_batch_render(prt_hip_ctx* ctx, prt_batch_stats* out)
+{
+    if (use_device(ctx)) return 1;
+    if (ctx->batch_jobs.empty()) return set_error(ctx, "batch_render: no jobs");
+    if (ensure_trace_resources(ctx)) return 1;
+    const bool counting = (ctx->profiling & PRT_PROFILE_COUNTERS) != 0;
+    const bool graph = ctx->batch_use_graph && !counting;
+    if (graph && !ctx->batch_graph_exec) {
+        PRT_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = batch_enqueue(ctx, ctx->batch_tuning, false, true);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        PRT_TRY(ctx, e);
+        ctx->batch_graph = g;
+        PRT_TRY(ctx, hipGraphInstantiate(&ctx->batch_graph_exec, g, nullptr, nullptr, 0));
+    }
+    PRT_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (graph) {
+        PRT_TRY(ctx, hipGraphLaunch(ctx->batch_graph_exec, ctx->stream));
+    } else if (batch_enqueue(ctx, ctx->batch_tuning, counting, true)) {
+        return 1;
+    }
+    PRT_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    unsigned long long st[8];
+    uint32_t err = 0;
+    PRT_TRY(ctx, hipMemcpyAsync(st, ctx->d_stats, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    PRT_TRY(ctx, hipMemcpyAsync(&err, ctx->d_counters + 32, 4, hipMemcpyDeviceToHost, ctx->stream));
+    PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (err) return set_error(ctx, "batch_render: traversal stack overflow");
+    float ms = 0.0f;
+    PRT_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    int64_t paths = 0;
+    for (BatchJobDevice& j : ctx->batch_jobs) {
+        paths += (int64_t)j.spp * j.width * j.height;
+        j.pass0 += j.spp;
+    }
+    if (out) {
+        std::memset(out, 0, sizeof(*out));
+        out->rays = (int64_t)st[3];
+        out->paths = paths;
+        out->ms = ms;
+        out->mray = ms > 0.0f ? (double)st[3] / 1000.0 / ms : 0.0;
+        if (counting) {
+            out->nodes = (int64_t)st[0];
+            out->tris = (int64_t)st[1];
+        }
+    }
+    return 0;
+}
+
+int prt_hip_batch_accum_read(prt_hip_ctx* ctx, int job, float* dst)
+{
+    if (use_device(ctx)) return 1;
+    if (job < 0 || job >= (int)ctx->batch_jobs.size() || !dst) return set_error(ctx, "batch_accum_read: bad job");
+    const BatchJobDevice& j = ctx->batch_jobs[job];
+    PRT_TRY(ctx, hipMemcpy(dst, ctx->d_batch_accum + j.pixel_base, sizeof(float4) * (size_t)j.width * j.height,
+                           hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int prt_hip_batch_set_tuning(prt_hip_ctx* ctx, const prt_batch_tuning* t)
+{
+    if (use_device(ctx)) return 1;
+    if (!t) return set_error(ctx, "batch_set_tuning: null");
+    BatchTuning tu{t->variant, t->blocks_per_cu, t->refill};
+    if (batch_resolve_tuning(ctx, tu)) return 1;
+    ctx->batch_tuning = tu;
+    batch_drop_graph(ctx);
+    return 0;
+}
+
+int prt_hip_batch_get_tuning(prt_hip_ctx* ctx, prt_batch_tuning* t)
+{
+    if (!ctx || !t) return 1;
+    std::memset(t, 0, sizeof(*t));
+    t->variant = ctx->batch_tuning.variant;
+    t->block = kBatchVariants[t->variant].block;
+    t->stack = kBatchVariants[t->variant].stack;
+    t->blocks_per_cu = ctx->batch_tuning.blocks_per_cu;
+    t->refill = ctx->batch_tuning.refill;
+    return 0;
+}
+
+int prt_hip_batch_autotune(prt_hip_ctx* ctx, prt_batch_tuning* results, int max_results, int* n_results)
+{
+    if (use_device(ctx)) return 1;
+    if (ctx->batch_jobs.empty()) return set_error(ctx, "batch_autotune: no jobs");
+    if (ensure_trace_resources(ctx)) return 1;
+    std::vector<prt_batch_tuning> all;
+    BatchTuning best{};
+    double best_ms = 1e300;
+    const int refills[2] = {16, 32};
+    for (int v = 0; v < kBatchVariantCount; ++v) {
+        BatchTuning probe{v, 0, 24};
+        if (batch_resolve_tuning(ctx, probe)) return 1;
+        const int max_bpc = probe.blocks_per_cu;
+        const int bpcs[2] = {max_bpc, max_bpc > 1 ? (max_bpc + 1) / 2 : 0};
+        for (int b = 0; b < 2; ++b) {
+            if (bpcs[b] <= 0 || (b == 1 && bpcs[1] == bpcs[0])) continue;
+            for (int r = 0; r < 2; ++r) {
+                BatchTuning tu{v, bpcs[b], refills[r]};
+                double t_best = 1e300;
+                for (int rep = 0; rep < 2; ++rep) {
+                    PRT_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+                    if (batch_enqueue(ctx, tu, false, false)) return 1;
+                    PRT_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+                    PRT_TRY(ctx, hipEventSynchronize(ctx->ev1));
+                    float ms = 0.0f;
+                    PRT_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+                    t_best = ms < t_best ? ms : t_best;
+                }
+                prt_batch_tuning rt{};
+                rt.variant = v;
+                rt.block = kBatchVariants[v].block;
+                rt.stack = kBatchVariants[v].stack;
+                rt.blocks_per_cu = tu.blocks_per_cu;
+                rt.refill = tu.refill;
+                rt.ms = t_best;
+                all.push_back(rt);
+                if (t_best < best_ms) { best_ms = t_best; best = tu; }
+            }
+        }
+    }
+    uint32_t err = 0;
+    PRT_TRY(ctx, hipMemcpy(&err, ctx->d_counters + 32, 4, hipMemcpyDeviceToHost));
+    if (err) return set_error(ctx, "batch_autotune: traversal stack overflow");
+    ctx->batch_tuning = best;
+    batch_drop_graph(ctx);
+    if (n_results) *n_results = (int)all.size();
+    if (results)
+        for (int i = 0; i < (int)all.size() && i < max_results; ++i) results[i] = all[i];
+    return 0;
+}
+
+int prt_hip_batch_set_graph(prt_hip_ctx* ctx, int enable)
+{
+    if (!ctx) return 1;
+    ctx->batch_use_graph = enable != 0;
+    batch_drop_graph(ctx);
+    return 0;
+}
+
+namespace {
+
+PathLaunch frame_path_launch(prt_hip_ctx* ctx, int pass, int max_depth)
+{
+    PathLaunch p{};
+    p.cam = to_camera(ctx->cam);
+    p.width = ctx->width;
+    p.height = ctx->height;
+    p.tiles_x = (ctx->width + 7) / 8;
+    p.entries = (uint32_t)frame_entries(ctx);
+    p.pass = pass;
+    p.inv_w = 1.0f / (float)ctx->width;
+    p.inv_h = 1.0f / (float)ctx->height;
+    p.accum = ctx->d_accum;
+    p.max_depth = max_depth;
+    p.albedo = 0.0f;
+    p.path_counter = ctx->d_counters + 16;
+    p.depth_rays = ctx->d_stats + 16;
+    p.stats = ctx->d_stats;
+    p.albedo_rgb = ctx->d_albedo;
+    p.spp = 1;
+    p.inv_spp = 1.0f;
+    return p;
+}
+
+int albedo_ready(prt_hip_ctx* ctx, int spp, int max_depth, const char* what)
+{
+    if (!ctx->scene.bvh_ready) return set_error(ctx, std::string(what) + ": no BVH");
+    if (!ctx->cam_ready || !ctx->d_accum) return set_error(ctx, std::string(what) + ": no camera or frame");
+    if (!ctx->d_albedo) return set_error(ctx, std::string(what) + ": no per-triangle albedo");
+    if (spp < 1 || max_depth < 0 || max_depth + 1 >= kMaxQueueDepth)
+        return set_error(ctx, std::string(what) + ": spp must be >= 1 and max_depth in [0, 14]");
+    return ensure_trace_resources(ctx);
+}
+
+int read_path_stats(prt_hip_ctx* ctx, prt_render_stats* out, int64_t paths)
+{
+    unsigned long long st[8];
+    uint32_t err = 0;
+    PRT_TRY(ctx, hipMemcpyAsync(st, ctx->d_stats, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    PRT_TRY(ctx, hipMemcpyAsync(&err, ctx->d_counters + 32, 4, hipMemcpyDeviceToHost, ctx->stream));
+    PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (err) return set_error(ctx, "traversal stack overflow");
+    float ms = 0.0f;
+    PRT_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (out) {
+        std::memset(out, 0, sizeof(*out));
+        out->rays = (int64_t)st[3];
+        out->paths = paths;
+        out->ms = ms;
+        out->mray = ms > 0.0f ? (double)st[3] / 1000.0 / ms : 0.0;
+        out->trace_ms = ms;
+        if (ctx->profiling & PRT_PROFILE_COUNTERS) {
+            out->nodes = (int64_t)st[0];
+            out->tris = (int64_t)st[1];
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+int prt_hip_set_albedo_host(prt_hip_ctx* ctx, const float* rgb)
+{
+    if (use_device(ctx)) return 1;
+    if (!rgb || ctx->scene.tri_count <= 0) return set_error(ctx, "set_albedo: no scene or null array");
+    free_ptr(ctx->d_albedo_own);
+    ctx->d_albedo_own = nullptr;
+    PRT_TRY(ctx, hipMalloc(&ctx->d_albedo_own, sizeof(float) * 3 * ctx->scene.tri_count));
+    PRT_TRY(ctx, hipMemcpy(ctx->d_albedo_own, rgb, sizeof(float) * 3 * ctx->scene.tri_count, hipMemcpyHostToDevice));
+    ctx->d_albedo = ctx->d_albedo_own;
+    return 0;
+}
+
+int prt_hip_set_albedo_device(prt_hip_ctx* ctx, const float* d_rgb)
+{
+    if (use_device(ctx)) return 1;
+    if (!d_rgb) return set_error(ctx, "set_albedo_device: null pointer");
+    ctx->d_albedo = d_rgb;
+    return 0;
+}
+
+int prt_hip_render_albedo_image(prt_hip_ctx* ctx, int pass0, int spp, int max_depth, float* d_image,
+                                prt_render_stats* out)
+{
+    if (use_device(ctx)) return 1;
+    if (albedo_ready(ctx, spp, max_depth, "render_albedo_image")) return 1;
+    if (!d_image) return set_error(ctx, "render_albedo_image: null image");
+    const int64_t pixels = (int64_t)ctx->width * ctx->height;
+    PRT_TRY(ctx, hipMemsetAsync(ctx->d_accum, 0, sizeof(float4) * pixels, ctx->stream));
+    PRT_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    PRT_TRY(ctx, launch_frame_begin(ctx, (uint32_t)frame_entries(ctx)));
+    for (int s = 0; s < spp; ++s) {
+        PathLaunch p = frame_path_launch(ctx, pass0 + s, max_depth);
+        PRT_TRY(ctx, launch_zero_u32(ctx->d_counters + 16, 1, ctx->stream));
+        PRT_TRY(ctx, launch_paths_rgb(ctx, p, (ctx->profiling & PRT_PROFILE_COUNTERS) != 0));
+    }
+    PRT_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    PRT_TRY(ctx, launch_accum_mean_rgb(ctx, d_image));
+    return read_path_stats(ctx, out, pixels * spp);
+}
+
+int prt_hip_render_albedo_image_host(prt_hip_ctx* ctx, int pass0, int spp, int max_depth, float* image,
+                                     prt_render_stats* out)
+{
+    if (use_device(ctx)) return 1;
+    if (!image) return set_error(ctx, "render_albedo_image: null image");
+    const int64_t n = (int64_t)ctx->width * ctx->height * 3;
+    float* d = nullptr;
+    PRT_TRY(ctx, hipMalloc(&d, sizeof(float) * n));
+    int rc = prt_hip_render_albedo_image(ctx, pass0, spp, max_depth, d, out);
+    if (rc == 0) {
+        hipError_t e = hipMemcpy(image, d, sizeof(float) * n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = check_hip(ctx, e, "render_albedo_image");
+    }
+    free_ptr(d);
+    return rc;
+}
+
+int prt_hip_adjoint_albedo(prt_hip_ctx* ctx, int pass0, int spp, int max_depth, const float* d_dldi, float* d_grad,
+                           prt_render_stats* out)
+{
+    if (use_device(ctx)) return 1;
+    if (albedo_ready(ctx, spp, max_depth, "adjoint_albedo")) return 1;
+    if (!d_dldi || !d_grad) return set_error(ctx, "adjoint_albedo: null array");
+    if ((uint64_t)frame_entries(ctx) * (uint64_t)spp >= (uint64_t)UINT32_MAX - (1u << 20))
+        return set_error(ctx, "adjoint_albedo: too many paths for one launch");
+    PathLaunch p = frame_path_launch(ctx, pass0, max_depth);
+    p.spp = spp;
+    p.inv_spp = 1.0f / (float)spp;
+    p.dldi = d_dldi;
+    p.grad = d_grad;
+    PRT_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    PRT_TRY(ctx, launch_frame_begin(ctx, (uint32_t)frame_entries(ctx)));
+    PRT_TRY(ctx, launch_adjoint(ctx, p, (ctx->profiling & PRT_PROFILE_COUNTERS) != 0));
+    PRT_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    return read_path_stats(ctx, out, (int64_t)ctx->width * ctx->height * spp);
+}
+
+int prt_hip_adjoint_albedo_host(prt_hip_ctx* ctx, int pass0, int spp, int max_depth, const float* dldi, float* grad,
+                                prt_render_stats* out)
+{
+    if (use_device(ctx)) return 1;
+    if (!dldi || !grad) return set_error(ctx, "adjoint_albedo: null array");
+    const int64_t ni = (int64_t)ctx->width * ctx->height * 3, ng = ctx->scene.tri_count * 3;
+    float *di = nullptr, *dg = nullptr;
+    hipError_t e = hipMalloc(&di, sizeof(float) * ni);
+    if (e == hipSuccess) e = hipMalloc(&dg, sizeof(float) * ng);
+    if (e == hipSuccess) e = hipMemcpy(di, dldi, sizeof(float) * ni, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dg, grad, sizeof(float) * ng, hipMemcpyHostToDevice);
+    int rc = e == hipSuccess ? prt_hip_adjoint_albedo(ctx, pass0, spp, max_depth, di, dg, out)
+                             : check_hip(ctx, e, "adjoint_albedo");
+    if (rc == 0) {
+        e = hipMemcpy(grad, dg, sizeof(float) * ng, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = check_hip(ctx, e, "adjoint_albedo");
+    }
+    free_ptr(di);
+    free_ptr(dg);
+    return rc;
+}
+
 int prt_hip_accum_read(prt_hip_ctx* ctx, float* dst)
 {
     if (use_device(ctx)) return 1;
@@ -541,25 +1049,25 @@ int prt_hip_debug_raygen(prt_hip_ctx* ctx, int pass, prt_ray* rays, uint32_t* st
 int prt_hip_bvh_info(prt_hip_ctx* ctx, int64_t* node_count, int64_t* tri_count, int32_t* depth)
 {
     if (!ctx) return 1;
-    if (!ctx->bvh_ready) return set_error(ctx, "bvh_info: no BVH");
-    if (node_count) *node_count = ctx->node_count;
-    if (tri_count) *tri_count = ctx->tri_count;
-    if (depth) *depth = ctx->bvh_depth;
+    if (!ctx->scene.bvh_ready) return set_error(ctx, "bvh_info: no BVH");
+    if (node_count) *node_count = ctx->scene.node_count;
+    if (tri_count) *tri_count = ctx->scene.tri_count;
+    if (depth) *depth = ctx->scene.bvh_depth;
     return 0;
 }
 
 int prt_hip_bvh_read(prt_hip_ctx* ctx, void* nodes, float* tri_records, int32_t* prim_ids)
 {
     if (use_device(ctx)) return 1;
-    if (!ctx->bvh_ready) return set_error(ctx, "bvh_read: no BVH");
+    if (!ctx->scene.bvh_ready) return set_error(ctx, "bvh_read: no BVH");
     if (nodes)
-        PRT_TRY(ctx, hipMemcpyAsync(nodes, ctx->d_nodes, sizeof(Bvh8Node) * ctx->node_count,
+        PRT_TRY(ctx, hipMemcpyAsync(nodes, ctx->scene.d_nodes, sizeof(Bvh8Node) * ctx->scene.node_count,
                                     hipMemcpyDeviceToHost, ctx->stream));
     if (tri_records)
-        PRT_TRY(ctx, hipMemcpyAsync(tri_records, ctx->d_tris, sizeof(float) * 9 * ctx->tri_count,
+        PRT_TRY(ctx, hipMemcpyAsync(tri_records, ctx->scene.d_tris, sizeof(float) * 9 * ctx->scene.tri_count,
                                     hipMemcpyDeviceToHost, ctx->stream));
     if (prim_ids)
-        PRT_TRY(ctx, hipMemcpyAsync(prim_ids, ctx->d_prim_ids, sizeof(int32_t) * ctx->tri_count,
+        PRT_TRY(ctx, hipMemcpyAsync(prim_ids, ctx->scene.d_prim_ids, sizeof(int32_t) * ctx->scene.tri_count,
                                     hipMemcpyDeviceToHost, ctx->stream));
     PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return 0;

@@ -529,22 +529,22 @@ hipError_t scan_exclusive(const uint32_t* in, uint32_t* out, int64_t n, hipStrea
 
 }  // namespace
 
-int build_bvh8(prt_hip_ctx* ctx, prt_build_stats* stats)
+int build_bvh8(prt_hip_ctx* ctx, SceneBuffers& sb, prt_build_stats* stats)
 {
-    const int64_t n = ctx->tri_count;
+    const int64_t n = sb.tri_count;
     hipStream_t s = ctx->stream;
     if (n <= 0) return set_error(ctx, "bvh_build: scene has no triangles");
     if (n >= (int64_t)1 << 31) return set_error(ctx, "bvh_build: more than 2^31-1 triangles");
 
-    if (ctx->d_nodes) { (void)hipFree(ctx->d_nodes); ctx->d_nodes = nullptr; }
-    if (ctx->d_tris) { (void)hipFree(ctx->d_tris); ctx->d_tris = nullptr; }
-    if (ctx->d_prim_ids) { (void)hipFree(ctx->d_prim_ids); ctx->d_prim_ids = nullptr; }
-    ctx->bvh_ready = false;
+    if (sb.d_nodes) { (void)hipFree(sb.d_nodes); sb.d_nodes = nullptr; }
+    if (sb.d_tris) { (void)hipFree(sb.d_tris); sb.d_tris = nullptr; }
+    if (sb.d_prim_ids) { (void)hipFree(sb.d_prim_ids); sb.d_prim_ids = nullptr; }
+    sb.bvh_ready = false;
 
     const int64_t max_nodes = std::max<int64_t>(1, n - 1);
-    PRT_TRY(ctx, hipMalloc(&ctx->d_nodes, sizeof(Bvh8Node) * max_nodes));
-    PRT_TRY(ctx, hipMalloc(&ctx->d_tris, sizeof(float) * 9 * n));
-    PRT_TRY(ctx, hipMalloc(&ctx->d_prim_ids, sizeof(int32_t) * n));
+    PRT_TRY(ctx, hipMalloc(&sb.d_nodes, sizeof(Bvh8Node) * max_nodes));
+    PRT_TRY(ctx, hipMalloc(&sb.d_tris, sizeof(float) * 9 * n));
+    PRT_TRY(ctx, hipMalloc(&sb.d_prim_ids, sizeof(int32_t) * n));
 
     const int nsort = blocks_for(n, kSortTile);
     const int nred = std::min(blocks_for(n, kBlock), 1024);
@@ -583,11 +583,11 @@ int build_bvh8(prt_hip_ctx* ctx, prt_build_stats* stats)
         Bvh8Node root{};
         std::vector<int32_t> idx(3);
         std::vector<float> pos;
-        PRT_TRY(ctx, hipMemcpyAsync(idx.data(), ctx->d_idx, 12, hipMemcpyDeviceToHost, s));
+        PRT_TRY(ctx, hipMemcpyAsync(idx.data(), sb.d_idx, 12, hipMemcpyDeviceToHost, s));
         PRT_TRY(ctx, hipStreamSynchronize(s));
         pos.resize(9);
         for (int v = 0; v < 3; ++v)
-            PRT_TRY(ctx, hipMemcpyAsync(&pos[3 * v], ctx->d_pos + 3 * (int64_t)idx[v], 12,
+            PRT_TRY(ctx, hipMemcpyAsync(&pos[3 * v], sb.d_pos + 3 * (int64_t)idx[v], 12,
                                         hipMemcpyDeviceToHost, s));
         PRT_TRY(ctx, hipStreamSynchronize(s));
         float lo[3], hi[3];
@@ -616,15 +616,15 @@ int build_bvh8(prt_hip_ctx* ctx, prt_build_stats* stats)
         root.child_base = 1;
         root.tri_base = 0;
         int32_t zero = 0;
-        PRT_TRY(ctx, hipMemcpyAsync(ctx->d_nodes, &root, sizeof(root), hipMemcpyHostToDevice, s));
-        PRT_TRY(ctx, hipMemcpyAsync(ctx->d_tris, pos.data(), 36, hipMemcpyHostToDevice, s));
-        PRT_TRY(ctx, hipMemcpyAsync(ctx->d_prim_ids, &zero, 4, hipMemcpyHostToDevice, s));
-        ctx->node_count = 1;
-        ctx->bvh_depth = 1;
+        PRT_TRY(ctx, hipMemcpyAsync(sb.d_nodes, &root, sizeof(root), hipMemcpyHostToDevice, s));
+        PRT_TRY(ctx, hipMemcpyAsync(sb.d_tris, pos.data(), 36, hipMemcpyHostToDevice, s));
+        PRT_TRY(ctx, hipMemcpyAsync(sb.d_prim_ids, &zero, 4, hipMemcpyHostToDevice, s));
+        sb.node_count = 1;
+        sb.bvh_depth = 1;
     } else {
-        k_centroid_partial<<<nred, kBlock, 0, s>>>(ctx->d_idx, ctx->d_pos, n, partial.as<float>());
+        k_centroid_partial<<<nred, kBlock, 0, s>>>(sb.d_idx, sb.d_pos, n, partial.as<float>());
         k_centroid_final<<<1, kBlock, 0, s>>>(partial.as<float>(), nred, bounds.as<float>());
-        k_morton<<<blocks_for(n, kBlock), kBlock, 0, s>>>(ctx->d_idx, ctx->d_pos, n, bounds.as<float>(),
+        k_morton<<<blocks_for(n, kBlock), kBlock, 0, s>>>(sb.d_idx, sb.d_pos, n, bounds.as<float>(),
                                                           keys[0].as<uint64_t>(), vals[0].as<uint32_t>());
         PRT_TRY(ctx, hipGetLastError());
         int cur = 0;
@@ -647,7 +647,7 @@ int build_bvh8(prt_hip_ctx* ctx, prt_build_stats* stats)
             sk, n, left.as<int32_t>(), right.as<int32_t>(), first.as<int32_t>(), last.as<int32_t>(),
             iparent.as<int32_t>(), lparent.as<int32_t>());
         PRT_TRY(ctx, hipMemsetAsync(flags.p, 0, 4 * ni, s));
-        k_refit<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, sv, ctx->d_idx, ctx->d_pos, left.as<int32_t>(),
+        k_refit<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, sv, sb.d_idx, sb.d_pos, left.as<int32_t>(),
                                                           right.as<int32_t>(), iparent.as<int32_t>(),
                                                           lparent.as<int32_t>(), ibox.as<float>(),
                                                           flags.as<uint32_t>());
@@ -686,8 +686,8 @@ int build_bvh8(prt_hip_ctx* ctx, prt_build_stats* stats)
             k_collapse_emit<<<nb, kBlock, 0, s>>>(
                 tasks[tc].as<int32_t>(), (int)ntasks, slots.as<int32_t>(), scan_int.as<uint32_t>(),
                 scan_tri.as<uint32_t>(), node_base, next_base, tri_base, first.as<int32_t>(),
-                last.as<int32_t>(), ibox.as<float>(), sv, ctx->d_idx, ctx->d_pos, ctx->d_nodes,
-                ctx->d_tris, ctx->d_prim_ids, tasks[1 - tc].as<int32_t>());
+                last.as<int32_t>(), ibox.as<float>(), sv, sb.d_idx, sb.d_pos, sb.d_nodes,
+                sb.d_tris, sb.d_prim_ids, tasks[1 - tc].as<int32_t>());
             PRT_TRY(ctx, hipGetLastError());
             node_base = next_base;
             tri_base += level_tris;
@@ -696,21 +696,21 @@ int build_bvh8(prt_hip_ctx* ctx, prt_build_stats* stats)
             ++depth;
         }
         if (tri_base != n) return set_error(ctx, "bvh_build: triangle records do not cover the scene");
-        ctx->node_count = node_base;
-        ctx->bvh_depth = depth;
+        sb.node_count = node_base;
+        sb.bvh_depth = depth;
     }
 
     PRT_TRY(ctx, hipEventRecord(ctx->ev1, s));
     PRT_TRY(ctx, hipEventSynchronize(ctx->ev1));
     float ms = 0.0f;
     PRT_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    ctx->bvh_ready = true;
+    sb.bvh_ready = true;
     if (stats) {
         stats->build_ms = ms;
         stats->build_mprim = ms > 0.0f ? (double)n / 1000.0 / ms : 0.0;
-        stats->node_count = ctx->node_count;
+        stats->node_count = sb.node_count;
         stats->tri_count = n;
-        stats->depth = ctx->bvh_depth;
+        stats->depth = sb.bvh_depth;
         stats->reserved = 0;
     }
     return 0;

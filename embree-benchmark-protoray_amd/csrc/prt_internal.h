@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <vector>
 
 #include "prt_math.h"
 #include "../../include/prt_hip.h"
@@ -45,6 +46,58 @@ struct SceneDevice {
     int64_t tri_count;
 };
 
+// One scene on the device: the uploaded mesh and its BVH8.
+struct SceneBuffers {
+    int64_t tri_count = 0, vertex_count = 0;
+    int32_t* d_idx = nullptr;
+    float* d_pos = nullptr;
+    Bvh8Node* d_nodes = nullptr;
+    float* d_tris = nullptr;
+    int32_t* d_prim_ids = nullptr;
+    int64_t node_count = 0;
+    int32_t bvh_depth = 0;
+    bool bvh_ready = false;
+    void release();
+    SceneDevice device() const { return SceneDevice{d_nodes, d_tris, d_prim_ids, tri_count}; }
+};
+
+// One job of a batch: a scene, a camera, an image and a sample count.
+struct BatchJobDevice {
+    Camera cam;
+    int32_t scene, width, height, tiles_x;
+    uint32_t entries;       // 8x8 tiles * 64 path slots per sample
+    int32_t spp, pass0, max_depth;
+    float albedo, inv_w, inv_h;
+    uint32_t work_begin;    // first work item (sample-major, then path)
+    uint64_t sample_base;   // float offset of the job's sample 0 in the sample planes
+    uint64_t pixel_base;    // float4 offset of the job's accumulator
+};
+
+struct BatchLaunch {
+    const BatchJobDevice* jobs;
+    const uint32_t* work_begin;  // n_jobs + 1 prefix of work items
+    int n_jobs;
+    const SceneDevice* scenes;
+    float* samples;              // path radiance per (job, sample, pixel)
+    uint32_t total_work;
+    uint32_t* work_counter;      // zero before launch
+    unsigned long long* stats;   // [0..2] instrumented counters, [3] rays
+};
+
+struct BatchKernelConfig {
+    int block, stack;
+};
+constexpr int kBatchVariantCount = 4;
+extern const BatchKernelConfig kBatchVariants[kBatchVariantCount];
+
+// Launch shape of the batch kernel: compiled variant (block size, LDS stack
+// depth), resident workgroups per CU (paths in flight) and refill threshold.
+struct BatchTuning {
+    int variant = 0;
+    int blocks_per_cu = 0;   // 0: the occupancy maximum
+    int refill = 24;
+};
+
 }  // namespace prt
 
 struct prt_hip_ctx {
@@ -54,18 +107,24 @@ struct prt_hip_ctx {
     int num_cus = 0;
     std::string device_name;
 
-    // scene
-    int64_t tri_count = 0, vertex_count = 0;
-    int32_t* d_idx = nullptr;
-    float* d_pos = nullptr;
+    // the scene of the plain (Device) path
+    prt::SceneBuffers scene;
 
-    // BVH8
-    prt::Bvh8Node* d_nodes = nullptr;
-    float* d_tris = nullptr;
-    int32_t* d_prim_ids = nullptr;
-    int64_t node_count = 0;
-    int32_t bvh_depth = 0;
-    bool bvh_ready = false;
+    // batched form: scenes, jobs and their buffers
+    std::vector<prt::SceneBuffers> batch_scenes;
+    std::vector<prt::BatchJobDevice> batch_jobs;    // host mirror (pass0 advanced per render)
+    prt::SceneDevice* d_batch_scenes = nullptr;
+    prt::BatchJobDevice* d_batch_jobs = nullptr;
+    uint32_t* d_batch_work_begin = nullptr;          // n+1
+    uint64_t* d_batch_pixel_begin = nullptr;         // n+1
+    float* d_batch_samples = nullptr;
+    float4* d_batch_accum = nullptr;
+    uint64_t batch_samples_len = 0, batch_pixels = 0;
+    uint32_t batch_total_work = 0;
+    prt::BatchTuning batch_tuning;
+    hipGraph_t batch_graph = nullptr;
+    hipGraphExec_t batch_graph_exec = nullptr;
+    bool batch_use_graph = true;
 
     // traversal spill stack (global part of the short stack)
     uint2* d_spill = nullptr;
@@ -98,13 +157,17 @@ struct prt_hip_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int profiling = 0;
     int pipeline = PRT_PIPELINE_FUSED;
+
+    // per-triangle RGB albedo: owned copy or an external device pointer
+    float* d_albedo_own = nullptr;
+    const float* d_albedo = nullptr;
     hipEvent_t kev[4 * 16] = {};    // per-launch timing events (trace/shade x begin/end)
 };
 
 namespace prt {
 
 // bvh_build.hip
-int build_bvh8(prt_hip_ctx* ctx, prt_build_stats* stats);
+int build_bvh8(prt_hip_ctx* ctx, SceneBuffers& sb, prt_build_stats* stats);
 
 // trace.hip
 struct TraceLaunch {
@@ -127,11 +190,23 @@ struct PathLaunch {
     float albedo;
     uint32_t* path_counter;          // zero before launch
     unsigned long long* depth_rays;  // [16] rays traced per depth, zero before launch
-    unsigned long long* stats;       // instrumented counters
+    unsigned long long* stats;       // instrumented counters; [3] rays
+    // per-triangle RGB albedo (RGB forward and adjoint modes), original triangle order
+    const float* albedo_rgb;
+    // adjoint: work items are (sample, path) for passes pass .. pass + spp - 1
+    int spp;
+    float inv_spp;
+    const float* dldi;               // dLoss/dImage, 3 floats per pixel
+    float* grad;                     // dLoss/dAlbedo, 3 floats per triangle (accumulated)
 };
 int ensure_trace_resources(prt_hip_ctx* ctx);
 hipError_t launch_trace(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumented);
 hipError_t launch_paths(prt_hip_ctx* ctx, const PathLaunch& p, bool primary, bool instrumented);
+hipError_t launch_paths_rgb(prt_hip_ctx* ctx, const PathLaunch& p, bool instrumented);
+hipError_t launch_adjoint(prt_hip_ctx* ctx, const PathLaunch& p, bool instrumented);
+// max_blocks_per_cu != nullptr: only query the variant's occupancy limit
+hipError_t launch_batch(prt_hip_ctx* ctx, const BatchLaunch& bl, const BatchTuning& tu, bool instrumented,
+                        int* max_blocks_per_cu);
 
 // render.hip
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t s);
@@ -142,6 +217,10 @@ hipError_t launch_shade_diffuse(prt_hip_ctx* ctx, int depth, int max_depth, floa
                                 float albedo_unused);
 hipError_t launch_shade_primary(prt_hip_ctx* ctx);
 hipError_t launch_tonemap(prt_hip_ctx* ctx);
+hipError_t launch_batch_begin(prt_hip_ctx* ctx);
+hipError_t launch_accum_mean_rgb(prt_hip_ctx* ctx, float* image);
+hipError_t launch_batch_finalize(prt_hip_ctx* ctx);
+hipError_t launch_batch_advance(prt_hip_ctx* ctx);
 hipError_t launch_unpack_rays(const float4* rays, const int32_t* pix, const uint32_t* rng,
                               int64_t n, int width, int height, prt_ray* out_rays,
                               uint32_t* out_states, hipStream_t s);

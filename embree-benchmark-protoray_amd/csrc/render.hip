@@ -209,6 +209,47 @@ __global__ __launch_bounds__(kBlock) void k_zero_u32(uint32_t* p, int n)
     if (i < n) p[i] = 0u;
 }
 
+// ---- batched form
+__global__ __launch_bounds__(64) void k_batch_begin(uint32_t* counters, unsigned long long* stats)
+{
+    if (threadIdx.x == 0) {
+        counters[32] = 0u;
+        counters[48] = 0u;
+    }
+    if (threadIdx.x < 8) stats[threadIdx.x] = 0ull;
+}
+
+// accum[job pixel] += the job's samples in pass order, so the result is the
+// same float sequence as one prt_hip_render call per sample.
+__global__ __launch_bounds__(kBlock) void k_batch_finalize(const BatchJobDevice* jobs,
+                                                           const uint64_t* pixel_begin, int n_jobs,
+                                                           const float* samples, float4* accum,
+                                                           uint64_t total_pixels)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= total_pixels) return;
+    int lo = 0, hi = n_jobs;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pixel_begin[mid] <= i) lo = mid; else hi = mid;
+    }
+    const BatchJobDevice& job = jobs[lo];
+    const uint64_t p = i - pixel_begin[lo];
+    const uint64_t plane = (uint64_t)job.width * (uint64_t)job.height;
+    float4 a = accum[i];
+    for (int k = 0; k < job.spp; ++k) {
+        const float v = samples[job.sample_base + (uint64_t)k * plane + p];
+        a = make_float4(a.x + v, a.y + v, a.z + v, a.w + 1.0f);
+    }
+    accum[i] = a;
+}
+
+// next render of the batch continues with the following passes
+__global__ __launch_bounds__(64) void k_batch_advance(BatchJobDevice* jobs, int n_jobs)
+{
+    for (int j = threadIdx.x; j < n_jobs; j += 64) jobs[j].pass0 += jobs[j].spp;
+}
+
 inline int persistent_blocks(prt_hip_ctx* ctx) { return ctx->num_cus * 8; }
 
 }  // namespace
@@ -247,21 +288,60 @@ hipError_t launch_frame_begin(prt_hip_ctx* ctx, uint32_t entries)
     return hipGetLastError();
 }
 
+static __global__ __launch_bounds__(kBlock) void k_accum_mean_rgb(const float4* accum, float* image, int n)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = accum[i];
+    const float inv = a.w > 0.0f ? 1.0f / a.w : 0.0f;
+    image[3 * i + 0] = a.x * inv;
+    image[3 * i + 1] = a.y * inv;
+    image[3 * i + 2] = a.z * inv;
+}
+
+hipError_t launch_accum_mean_rgb(prt_hip_ctx* ctx, float* image)
+{
+    const int n = ctx->width * ctx->height;
+    k_accum_mean_rgb<<<(n + kBlock - 1) / kBlock, kBlock, 0, ctx->stream>>>(ctx->d_accum, image, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_begin(prt_hip_ctx* ctx)
+{
+    k_batch_begin<<<1, 64, 0, ctx->stream>>>(ctx->d_counters, ctx->d_stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_finalize(prt_hip_ctx* ctx)
+{
+    const uint64_t n = ctx->batch_pixels;
+    k_batch_finalize<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(
+        ctx->d_batch_jobs, ctx->d_batch_pixel_begin, (int)ctx->batch_jobs.size(), ctx->d_batch_samples,
+        ctx->d_batch_accum, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_advance(prt_hip_ctx* ctx)
+{
+    k_batch_advance<<<1, 64, 0, ctx->stream>>>(ctx->d_batch_jobs, (int)ctx->batch_jobs.size());
+    return hipGetLastError();
+}
+
 hipError_t launch_shade_diffuse(prt_hip_ctx* ctx, int depth, int max_depth, float throughput, float)
 {
     const int cur = depth & 1;
     k_shade_diffuse<<<persistent_blocks(ctx), kBlock, 0, ctx->stream>>>(
         ctx->d_rays[cur], ctx->d_hits, ctx->d_pix[cur], ctx->d_rng[cur], ctx->d_counters + depth,
         ctx->d_rays[1 - cur], ctx->d_pix[1 - cur], ctx->d_rng[1 - cur], ctx->d_counters + depth + 1,
-        ctx->d_accum, ctx->d_idx, ctx->d_pos, throughput, depth == max_depth ? 1 : 0);
+        ctx->d_accum, ctx->scene.d_idx, ctx->scene.d_pos, throughput, depth == max_depth ? 1 : 0);
     return hipGetLastError();
 }
 
 hipError_t launch_shade_primary(prt_hip_ctx* ctx)
 {
     k_shade_primary<<<persistent_blocks(ctx), kBlock, 0, ctx->stream>>>(
-        ctx->d_rays[0], ctx->d_hits, ctx->d_pix[0], ctx->d_counters, ctx->d_accum, ctx->d_idx,
-        ctx->d_pos);
+        ctx->d_rays[0], ctx->d_hits, ctx->d_pix[0], ctx->d_counters, ctx->d_accum, ctx->scene.d_idx,
+        ctx->scene.d_pos);
     return hipGetLastError();
 }
 

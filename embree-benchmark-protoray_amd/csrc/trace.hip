@@ -36,10 +36,10 @@
 namespace prt {
 namespace {
 
-constexpr int kStackLds = 16;      // LDS entries per lane (power of two)
-constexpr int kRefill = 24;        // refill when this many lanes are idle
+constexpr int kStackLds = 16;      // default LDS entries per lane (power of two)
+constexpr int kRefill = 24;        // default: refill when this many lanes are idle
 
-enum { kModeTrace = 0, kModeDiffuse = 1, kModePrimary = 2 };
+enum { kModeTrace = 0, kModeDiffuse = 1, kModePrimary = 2, kModeBatch = 3, kModeRgb = 4, kModeAdjoint = 5 };
 
 __device__ __forceinline__ float byte_f(uint32_t w, int k)
 {
@@ -48,16 +48,17 @@ __device__ __forceinline__ float byte_f(uint32_t w, int k)
 
 __device__ __forceinline__ float as_f(uint32_t u) { return __uint_as_float(u); }
 
-template <bool kCount, int kMode>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaunch tl, PathLaunch pl,
-                                                       uint2* spill, int stack_entries, uint32_t* err)
+template <bool kCount, int kMode, int kBlock = kTraceBlock, int kStack = kStackLds>
+__global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl, PathLaunch pl, BatchLaunch bl,
+                                                  uint2* spill, int stack_entries, int refill, uint32_t* err)
 {
-    __shared__ uint32_t st_node[kStackLds * kTraceBlock];
-    __shared__ float st_t[kStackLds * kTraceBlock];
+    static_assert((kStack & (kStack - 1)) == 0, "LDS stack depth must be a power of two");
+    __shared__ uint32_t st_node[kStack * kBlock];
+    __shared__ float st_t[kStack * kBlock];
     // rays per depth: an LDS histogram in the instrumented variant only, so
     // the timed variant keeps 32 KiB of LDS per block (5 blocks per CU)
     uint32_t* depth_hist = nullptr;
-    if constexpr (kCount && kMode != kModeTrace) {
+    if constexpr (kCount && kMode != kModeTrace && kMode != kModeBatch) {
         __shared__ uint32_t hist[kMaxQueueDepth];
         depth_hist = hist;
     }
@@ -66,10 +67,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const uint64_t lane_lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t count = kMode == kModeTrace ? (tl.count_ptr ? *tl.count_ptr : tl.count) : pl.entries;
-    uint2* my_spill = spill + ((size_t)blockIdx.x * kTraceBlock + tid) * (size_t)stack_entries;
+    const uint32_t count = kMode == kModeTrace ? (tl.count_ptr ? *tl.count_ptr : tl.count)
+                         : (kMode == kModeBatch ? bl.total_work
+                                                : (kMode == kModeAdjoint ? pl.entries * (uint32_t)pl.spp : pl.entries));
+    uint2* my_spill = spill + ((size_t)blockIdx.x * kBlock + tid) * (size_t)stack_entries;
     const int spill_cap = stack_entries;
-    if constexpr (kCount && kMode != kModeTrace) {
+    if constexpr (kCount && kMode != kModeTrace && kMode != kModeBatch) {
         if (tid < kMaxQueueDepth) depth_hist[tid] = 0;
         __syncthreads();
     }
@@ -91,6 +94,23 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
     uint32_t rng = 0;
     int depth = 0;
     float lw = 1.0f;
+    int max_depth = pl.max_depth;
+    float albedo = pl.albedo;
+    // scene of the lane's current ray (per lane in the batch mode)
+    const Bvh8Node* nodes = sc.nodes;
+    const float* tris = sc.tris;
+    const int32_t* prim_ids = sc.prim_ids;
+    float* sample_out = nullptr;
+    // RGB throughput (RGB and adjoint modes); adjoint: per-channel product of
+    // the path's nonzero albedos and count of zero albedos from the primal
+    // pass, and the path's weight dLoss/dI / spp
+    float lg = 1.0f, lb = 1.0f;
+    float pr = 1.0f, pg = 1.0f, pb = 1.0f;
+    uint32_t zeros = 0;  // 8 bits per channel
+    float gw_r = 0.0f, gw_g = 0.0f, gw_b = 0.0f;
+    bool replay = false;
+    int path_pass = 0;
+    int path_x = 0, path_y = 0;
     unsigned long long n_nodes = 0, n_tris = 0, n_spill = 0;
     bool exhausted = false;  // wave-uniform: the work counter has run past count
 
@@ -119,11 +139,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
         // ---- refill idle lanes (wave-uniform branch)
         const uint64_t idle = __ballot(!active);
         const int n_idle = __popcll(idle);
-        if (!exhausted && n_idle >= kRefill) {
+        if (!exhausted && n_idle >= refill) {
             const int leader = __ffsll((unsigned long long)idle) - 1;
             uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(kMode == kModeTrace ? tl.fetch_ptr : pl.path_counter,
-                                                 (uint32_t)n_idle);
+            if (lane == leader)
+                base = atomicAdd(kMode == kModeTrace ? tl.fetch_ptr
+                                                     : (kMode == kModeBatch ? bl.work_counter : pl.path_counter),
+                                 (uint32_t)n_idle);
             base = __shfl(base, leader, 64);
             if (!active) {
                 const uint32_t my = base + (uint32_t)__popcll(idle & lane_lt);
@@ -138,17 +160,64 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
                             start_ray(mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), r0.w, r1.w);
                             active = true;
                         }
+                    } else if (kMode == kModeBatch) {
+                        // work item -> job (binary search over the job prefix),
+                        // then sample and path inside the job
+                        int lo = 0, hi = bl.n_jobs;
+                        while (hi - lo > 1) {
+                            const int mid = (lo + hi) >> 1;
+                            if (bl.work_begin[mid] <= my) lo = mid; else hi = mid;
+                        }
+                        const BatchJobDevice& job = bl.jobs[lo];
+                        const uint32_t local = my - bl.work_begin[lo];
+                        const uint32_t sample = local / job.entries;
+                        const uint32_t path = local - sample * job.entries;
+                        const uint32_t tile = path >> 6, tl_lane = path & 63u;
+                        const int mx = (tl_lane & 1) | ((tl_lane >> 1) & 2) | ((tl_lane >> 2) & 4);
+                        const int my_ = ((tl_lane >> 1) & 1) | ((tl_lane >> 2) & 2) | ((tl_lane >> 3) & 4);
+                        const int x = (int)(tile % (uint32_t)job.tiles_x) * 8 + mx;
+                        const int y = (int)(tile / (uint32_t)job.tiles_x) * 8 + my_;
+                        if (x < job.width && y < job.height) {
+                            pixel = x + y * job.width;
+                            const SceneDevice jsc = bl.scenes[job.scene];
+                            nodes = jsc.nodes;
+                            tris = jsc.tris;
+                            prim_ids = jsc.prim_ids;
+                            sample_out = bl.samples + job.sample_base +
+                                         (size_t)sample * (size_t)(job.width * job.height) + pixel;
+                            max_depth = job.max_depth;
+                            albedo = job.albedo;
+                            rng = sampler_seed(job.pass0 + (int)sample, pixel);
+                            const float u1 = sampler_next(rng);
+                            const float u2 = sampler_next(rng);
+                            (void)sampler_next(rng);  // lens 2D
+                            (void)sampler_next(rng);
+                            const f3 dir = camera_dir(job.cam, ((float)x + u1) * job.inv_w, ((float)y + u2) * job.inv_h);
+                            start_ray(job.cam.origin, dir, 0.0f, 3.402823466e38f);
+                            depth = 0;
+                            lw = 1.0f;
+                            active = true;
+                        }
                     } else {
                         // camera ray of path `my`: 8x8 tiles, Morton order inside
                         // a tile (image/morton.cpp:22), as the wavefront raygen
-                        const uint32_t tile = my >> 6, tl_lane = my & 63u;
+                        uint32_t path = my;
+                        path_pass = pl.pass;
+                        if (kMode == kModeAdjoint) {
+                            const uint32_t sample = my / pl.entries;
+                            path = my - sample * pl.entries;
+                            path_pass = pl.pass + (int)sample;
+                        }
+                        const uint32_t tile = path >> 6, tl_lane = path & 63u;
                         const int mx = (tl_lane & 1) | ((tl_lane >> 1) & 2) | ((tl_lane >> 2) & 4);
                         const int my_ = ((tl_lane >> 1) & 1) | ((tl_lane >> 2) & 2) | ((tl_lane >> 3) & 4);
                         const int x = (int)(tile % (uint32_t)pl.tiles_x) * 8 + mx;
                         const int y = (int)(tile / (uint32_t)pl.tiles_x) * 8 + my_;
                         if (x < pl.width && y < pl.height) {
                             pixel = x + y * pl.width;
-                            rng = sampler_seed(pl.pass, pixel);
+                            path_x = x;
+                            path_y = y;
+                            rng = sampler_seed(path_pass, pixel);
                             const float u1 = sampler_next(rng);
                             const float u2 = sampler_next(rng);
                             (void)sampler_next(rng);  // lens 2D
@@ -156,8 +225,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
                             const f3 dir = camera_dir(pl.cam, ((float)x + u1) * pl.inv_w, ((float)y + u2) * pl.inv_h);
                             start_ray(pl.cam.origin, dir, 0.0f, 3.402823466e38f);
                             depth = 0;
-                            lw = 1.0f;
+                            lw = lg = lb = 1.0f;
                             active = true;
+                            if (kMode == kModeAdjoint) {
+                                replay = false;
+                                pr = pg = pb = 1.0f;
+                                zeros = 0;
+                                const float* g = pl.dldi + 3 * (size_t)pixel;
+                                gw_r = g[0] * pl.inv_spp;
+                                gw_g = g[1] * pl.inv_spp;
+                                gw_b = g[2] * pl.inv_spp;
+                                if (gw_r == 0.0f && gw_g == 0.0f && gw_b == 0.0f) active = false;
+                            }
                         }
                     }
                 }
@@ -183,7 +262,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
                         nid = e.x;
                         nt = as_f(e.y);
                     } else {
-                        const int slot = (sp & (kStackLds - 1)) * kTraceBlock + tid;
+                        const int slot = (sp & (kStack - 1)) * kBlock + tid;
                         nid = st_node[slot];
                         nt = st_t[slot];
                     }
@@ -194,7 +273,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
                     float t = -1.0f;
                     f3 p0{}, p1{}, p2{};
                     if (best_prim >= 0) {
-                        const float* tp = sc.tris + 9 * (size_t)best_rec;
+                        const float* tp = tris + 9 * (size_t)best_rec;
                         p0 = mk3(tp[0], tp[1], tp[2]);
                         p1 = mk3(tp[3], tp[4], tp[5]);
                         p2 = mk3(tp[6], tp[7], tp[8]);
@@ -207,9 +286,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
                         active = false;
                         break;
                     }
-                    if constexpr (kCount) atomicAdd(&depth_hist[depth], 1u);
+                    if constexpr (kCount && kMode != kModeBatch) atomicAdd(&depth_hist[depth], 1u);
                     ++n_rays;
-                    float4* acc = pl.accum + pixel;
+                    float4* acc = kMode == kModeBatch ? nullptr : pl.accum + pixel;
                     if (kMode == kModePrimary) {
                         f3 c = mk3(0.05f, 0.05f, 0.05f);
                         if (best_prim >= 0) {
@@ -221,15 +300,69 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
                         active = false;
                         break;
                     }
-                    if (best_prim < 0 || depth == pl.max_depth) {
+                    if (best_prim < 0 || depth == max_depth) {
+                        if (kMode == kModeAdjoint) {
+                            if (!replay && best_prim < 0) {
+                                // the path escaped: replay it from its seed to
+                                // hand each hit its share of the gradient
+                                replay = true;
+                                rng = sampler_seed(path_pass, pixel);
+                                const float u1 = sampler_next(rng);
+                                const float u2 = sampler_next(rng);
+                                (void)sampler_next(rng);
+                                (void)sampler_next(rng);
+                                const f3 dir = camera_dir(pl.cam, ((float)path_x + u1) * pl.inv_w,
+                                                          ((float)path_y + u2) * pl.inv_h);
+                                depth = 0;
+                                start_ray(pl.cam.origin, dir, 0.0f, 3.402823466e38f);
+                                continue;
+                            }
+                            active = false;
+                            break;
+                        }
                         const float c = best_prim < 0 ? lw : 0.0f;
-                        const float4 a = *acc;
-                        *acc = make_float4(a.x + c, a.y + c, a.z + c, a.w + 1.0f);
+                        if (kMode == kModeBatch) {
+                            *sample_out = c;
+                        } else if (kMode == kModeRgb) {
+                            const float cg = best_prim < 0 ? lg : 0.0f, cb = best_prim < 0 ? lb : 0.0f;
+                            const float4 a = *acc;
+                            *acc = make_float4(a.x + c, a.y + cg, a.z + cb, a.w + 1.0f);
+                        } else {
+                            const float4 a = *acc;
+                            *acc = make_float4(a.x + c, a.y + c, a.z + c, a.w + 1.0f);
+                        }
                         active = false;
                         break;
                     }
                     // diffuse bounce: the next ray of this path
-                    lw *= pl.albedo;
+                    if (kMode == kModeRgb || kMode == kModeAdjoint) {
+                        const float* a = pl.albedo_rgb + 3 * (size_t)best_prim;
+                        const float ar = a[0], ag = a[1], ab = a[2];
+                        if (kMode == kModeAdjoint) {
+                            if (!replay) {
+                                // primal: products of the nonzero albedos, zero counts
+                                if (ar != 0.0f) pr *= ar; else zeros += 1u;
+                                if (ag != 0.0f) pg *= ag; else zeros += 1u << 8;
+                                if (ab != 0.0f) pb *= ab; else zeros += 1u << 16;
+                            } else {
+                                // dL_c/da_c(t) = product of the path's other albedos
+                                float* g = pl.grad + 3 * (size_t)best_prim;
+                                const uint32_t zr = zeros & 0xffu, zg = (zeros >> 8) & 0xffu, zb = zeros >> 16;
+                                const float dr = zr == 0 ? pr / ar : (zr == 1 && ar == 0.0f ? pr : 0.0f);
+                                const float dg = zg == 0 ? pg / ag : (zg == 1 && ag == 0.0f ? pg : 0.0f);
+                                const float db = zb == 0 ? pb / ab : (zb == 1 && ab == 0.0f ? pb : 0.0f);
+                                if (gw_r * dr != 0.0f) atomicAdd(g + 0, gw_r * dr);
+                                if (gw_g * dg != 0.0f) atomicAdd(g + 1, gw_g * dg);
+                                if (gw_b * db != 0.0f) atomicAdd(g + 2, gw_b * db);
+                            }
+                        } else {
+                            lw *= ar;
+                            lg *= ag;
+                            lb *= ab;
+                        }
+                    } else {
+                        lw *= albedo;
+                    }
                     const f3 n = face_normal(p0, p1, p2, d);
                     const float s1 = sampler_next(rng);
                     const float s2 = sampler_next(rng);
@@ -241,7 +374,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
             }
 
             // ---- visit one BVH8 node
-            const uint4* np = reinterpret_cast<const uint4*>(sc.nodes + node);
+            const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
             const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
             if (kCount) ++n_nodes;
             const int ex = (int)(int8_t)(w0.w & 0xffu);
@@ -304,7 +437,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
                 const uint32_t cnt = mb >> 5;
                 for (uint32_t j = 0; j < cnt; ++j) {
                     const uint32_t rec = first + j;
-                    const float* tp = sc.tris + 9 * (size_t)rec;
+                    const float* tp = tris + 9 * (size_t)rec;
                     const f3 p0 = mk3(tp[0], tp[1], tp[2]);
                     const f3 p1 = mk3(tp[3], tp[4], tp[5]);
                     const f3 p2 = mk3(tp[6], tp[7], tp[8]);
@@ -312,7 +445,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
                     TriCandidate c;
                     if (tri_test(wr, o, p0, p1, p2, tmin, tbest, c)) {
                         if (c.t <= tbest) {
-                            const int32_t prim = sc.prim_ids[rec];
+                            const int32_t prim = prim_ids[rec];
                             if (c.t < tbest || best_prim < 0 || prim < best_prim) {
                                 tbest = c.t;
                                 best_prim = prim;
@@ -344,21 +477,21 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
                 } else {
                     push_n = cidx; push_t = t;
                 }
-                if (sp - bottom == kStackLds) {
+                if (sp - bottom == kStack) {
                     if (bottom >= spill_cap) { atomicOr(err, 1u); break; }
-                    const int bslot = (bottom & (kStackLds - 1)) * kTraceBlock + tid;
+                    const int bslot = (bottom & (kStack - 1)) * kBlock + tid;
                     my_spill[bottom] = make_uint2(st_node[bslot], __float_as_uint(st_t[bslot]));
                     ++bottom;
                     if (kCount) ++n_spill;
                 }
-                const int slot = (sp & (kStackLds - 1)) * kTraceBlock + tid;
+                const int slot = (sp & (kStack - 1)) * kBlock + tid;
                 st_node[slot] = (uint32_t)push_n;
                 st_t[slot] = push_t;
                 ++sp;
             }
             node = next;
 
-            if (!exhausted && __popcll(__ballot(1)) <= 64 - kRefill) break;
+            if (!exhausted && __popcll(__ballot(1)) <= 64 - refill) break;
         }
     }
 
@@ -369,7 +502,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
             n_spill += __shfl_down(n_spill, off, 64);
         }
         if (lane == 0) {
-            unsigned long long* st = kMode == kModeTrace ? tl.stats : pl.stats;
+            unsigned long long* st = kMode == kModeTrace ? tl.stats : (kMode == kModeBatch ? bl.stats : pl.stats);
             atomicAdd(&st[0], n_nodes);
             atomicAdd(&st[1], n_tris);
             atomicAdd(&st[2], n_spill);
@@ -378,8 +511,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
     if constexpr (kMode != kModeTrace) {
         unsigned long long r = n_rays;
         for (int off = 32; off > 0; off >>= 1) r += __shfl_down(r, off, 64);
-        if (lane == 0 && r) atomicAdd(&pl.stats[3], r);
-        if constexpr (kCount) {
+        if (lane == 0 && r) atomicAdd(&(kMode == kModeBatch ? bl.stats : pl.stats)[3], r);
+        if constexpr (kCount && kMode != kModeBatch) {
             __syncthreads();
             if (tid < kMaxQueueDepth && depth_hist[tid])
                 atomicAdd(&pl.depth_rays[tid], (unsigned long long)depth_hist[tid]);
@@ -390,14 +523,41 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(SceneDevice sc, TraceLaun
 template <int kMode>
 hipError_t launch_mode(prt_hip_ctx* ctx, const TraceLaunch& t, const PathLaunch& p, bool instrumented)
 {
-    SceneDevice sc{ctx->d_nodes, ctx->d_tris, ctx->d_prim_ids, ctx->tri_count};
+    const SceneDevice sc = ctx->scene.device();
     uint32_t* err = ctx->d_counters + 32;
+    const BatchLaunch bl{};
     if (instrumented)
-        k_trace<true, kMode><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(sc, t, p, ctx->d_spill,
-                                                                               ctx->stack_entries, err);
+        k_trace<true, kMode><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(
+            sc, t, p, bl, ctx->d_spill, ctx->stack_entries, kRefill, err);
     else
-        k_trace<false, kMode><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(sc, t, p, ctx->d_spill,
-                                                                                ctx->stack_entries, err);
+        k_trace<false, kMode><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(
+            sc, t, p, bl, ctx->d_spill, ctx->stack_entries, kRefill, err);
+    return hipGetLastError();
+}
+
+// Batch kernel variants the autotuner chooses from: (block size, LDS stack).
+template <int kBlock, int kStack>
+hipError_t launch_batch_variant(prt_hip_ctx* ctx, const BatchLaunch& bl, int blocks_per_cu, int refill,
+                                bool instrumented, int* max_blocks_per_cu)
+{
+    if (max_blocks_per_cu) {
+        int per_cu = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, k_trace<false, kModeBatch, kBlock, kStack>, kBlock, 0);
+        *max_blocks_per_cu = per_cu;
+        return e;
+    }
+    const int grid = blocks_per_cu * ctx->num_cus;
+    uint32_t* err = ctx->d_counters + 32;
+    const SceneDevice sc{};
+    const TraceLaunch t{};
+    const PathLaunch p{};
+    if (instrumented)
+        k_trace<true, kModeBatch, kBlock, kStack><<<grid, kBlock, 0, ctx->stream>>>(
+            sc, t, p, bl, ctx->d_spill, ctx->stack_entries, refill, err);
+    else
+        k_trace<false, kModeBatch, kBlock, kStack><<<grid, kBlock, 0, ctx->stream>>>(
+            sc, t, p, bl, ctx->d_spill, ctx->stack_entries, refill, err);
     return hipGetLastError();
 }
 
@@ -409,13 +569,19 @@ int ensure_trace_resources(prt_hip_ctx* ctx)
         int per_cu = 0;
         PRT_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, kModeDiffuse>,
                                                                   kTraceBlock, 0));
+        if (per_cu * kTraceBlock > 2048) per_cu = 2048 / kTraceBlock;
         if (per_cu < 1) per_cu = 1;
         ctx->trace_grid = per_cu * ctx->num_cus;
     }
-    // worst case stack: 7 pushes per level of the tree, plus slack
-    int need = 7 * (ctx->bvh_depth + 1) + 8;
+    // worst case stack: 7 pushes per level of the tree, plus slack; sized for
+    // the deepest scene and for every lane a CU can hold (2048), so any
+    // persistent launch shape fits
+    int depth = ctx->scene.bvh_depth;
+    for (const SceneBuffers& b : ctx->batch_scenes) depth = depth > b.bvh_depth ? depth : b.bvh_depth;
+    int need = 7 * (depth + 1) + 8;
     if (need < kStackLds) need = kStackLds;
-    const int64_t bytes = (int64_t)ctx->trace_grid * kTraceBlock * need * (int64_t)sizeof(uint2);
+    const int64_t lanes = (int64_t)ctx->num_cus * 2048;
+    const int64_t bytes = lanes * need * (int64_t)sizeof(uint2);
     if (bytes > ctx->spill_bytes || need != ctx->stack_entries) {
         if (ctx->d_spill) { (void)hipFree(ctx->d_spill); ctx->d_spill = nullptr; }
         PRT_TRY(ctx, hipMalloc(&ctx->d_spill, bytes));
@@ -430,11 +596,37 @@ hipError_t launch_trace(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumente
     return launch_mode<kModeTrace>(ctx, t, PathLaunch{}, instrumented);
 }
 
+hipError_t launch_paths_rgb(prt_hip_ctx* ctx, const PathLaunch& p, bool instrumented)
+{
+    return launch_mode<kModeRgb>(ctx, TraceLaunch{}, p, instrumented);
+}
+
+hipError_t launch_adjoint(prt_hip_ctx* ctx, const PathLaunch& p, bool instrumented)
+{
+    return launch_mode<kModeAdjoint>(ctx, TraceLaunch{}, p, instrumented);
+}
+
 hipError_t launch_paths(prt_hip_ctx* ctx, const PathLaunch& p, bool primary, bool instrumented)
 {
     TraceLaunch t{};
     return primary ? launch_mode<kModePrimary>(ctx, t, p, instrumented)
                    : launch_mode<kModeDiffuse>(ctx, t, p, instrumented);
+}
+
+const BatchKernelConfig kBatchVariants[kBatchVariantCount] = {
+    {256, 16}, {256, 8}, {128, 16}, {128, 8},
+};
+
+hipError_t launch_batch(prt_hip_ctx* ctx, const BatchLaunch& bl, const BatchTuning& tu, bool instrumented,
+                        int* max_blocks_per_cu)
+{
+    switch (tu.variant) {
+    case 0: return launch_batch_variant<256, 16>(ctx, bl, tu.blocks_per_cu, tu.refill, instrumented, max_blocks_per_cu);
+    case 1: return launch_batch_variant<256, 8>(ctx, bl, tu.blocks_per_cu, tu.refill, instrumented, max_blocks_per_cu);
+    case 2: return launch_batch_variant<128, 16>(ctx, bl, tu.blocks_per_cu, tu.refill, instrumented, max_blocks_per_cu);
+    case 3: return launch_batch_variant<128, 8>(ctx, bl, tu.blocks_per_cu, tu.refill, instrumented, max_blocks_per_cu);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace prt

@@ -252,6 +252,11 @@ CONFIGS = {
         "view": {"pos": [0.0, 4.5, 1.2], "angle_x": 1.3, "angle_y": 0.0, "fov_y": math.radians(45.0)},
         "width": 512, "height": 512, "spp": 4, "max_depth": 4, "albedo": 0.8,
     },
+    "heightfield_boxes_512": {
+        "scene": "heightfield_boxes", "seed": 1004, "albedo_seed": 3004, "target_seed": 4004,
+        "view": {"pos": [0.0, 1.9, 2.1], "angle_x": 0.75, "angle_y": 0.0, "fov_y": math.radians(50.0)},
+        "width": 512, "height": 512, "spp": 8, "max_depth": 6, "albedo": None,
+    },
     "spheres_1080p": {
         "scene": "spheres", "seed": 1002,
         "view": {"pos": [0.5, 0.9, 2.6], "angle_x": 0.25, "angle_y": 0.0, "fov_y": math.radians(45.0)},
@@ -260,10 +265,69 @@ CONFIGS = {
 }
 
 
+def scene_mix(seed: int, target_tris: int) -> Mesh:
+    """A heightfield on [-1,1]^2 plus icospheres, about `target_tris` triangles
+    split at a random fraction between the two (the batch workload)."""
+    rng = np.random.default_rng(seed)
+    frac = rng.uniform(0.2, 0.8)
+    hf_tris = max(8, int(frac * target_tris))
+    cells = max(2, int(round(math.sqrt(hf_tris / 2.0))))
+    parts = [heightfield(cells, rng, rng.uniform(0.05, 0.3))]
+    rest = max(80, target_tris - 2 * cells * cells)
+    subdiv = int(np.clip(math.floor(math.log(max(rest / 160.0, 1.0), 4)), 0, 4))
+    f, v = icosphere(subdiv)
+    count = max(1, int(round(rest / len(f))))
+    centres = np.stack([rng.uniform(-1, 1, count), rng.uniform(0.15, 0.9, count), rng.uniform(-1, 1, count)], 1)
+    radii = np.exp(rng.uniform(math.log(0.02), math.log(0.15), count))
+    parts += [(f, (v * r + c).astype(np.float32)) for c, r in zip(centres, radii)]
+    return merge(parts)
+
+
+def scene_heightfield_boxes(seed: int = 1004, cells: int = 128, boxes: int = 64) -> Mesh:
+    """The differentiable workload: a `cells` x `cells` heightfield (heights
+    U[0, 0.2]) on [-1,1]^2 plus `boxes` closed boxes (12 triangles each) standing
+    on it, footprints U[0.05, 0.25], heights U[0.1, 0.6]."""
+    rng = np.random.default_rng(seed)
+    parts = [heightfield(cells, rng, 0.2)]
+    for _ in range(boxes):
+        cx, cz = rng.uniform(-0.9, 0.9, 2)
+        sx, sz = rng.uniform(0.05, 0.25, 2)
+        hgt = rng.uniform(0.1, 0.6)
+        parts.append(box_tris((cx - sx / 2, 0.0, cz - sz / 2), (cx + sx / 2, hgt, cz + sz / 2)))
+    return merge(parts)
+
+
+def albedo_set(tri_count: int, seed: int, lo: float = 0.1, hi: float = 0.9) -> np.ndarray:
+    """Per-triangle RGB albedo, uniform in [lo, hi], shape (tri_count, 3)."""
+    return np.random.default_rng(seed).uniform(lo, hi, size=(tri_count, 3)).astype(np.float32)
+
+
+BATCH_RESOLUTIONS = [(256, 256), (640, 480), (1280, 720), (1920, 1080)]
+BATCH_VIEW = {"pos": [0.0, 1.7, 2.7], "angle_x": 0.55, "angle_y": 0.0, "fov_y": math.radians(50.0)}
+
+
+def batch_workload(n: int = 32, seed0: int = 2000, min_tris: float = 1e3, max_tris: float = 2e6,
+                   max_depth: int = 6) -> list[dict]:
+    """The batched workload: n scenes with triangle counts log-uniform in
+    [min_tris, max_tris], resolutions from BATCH_RESOLUTIONS, spp uniform in
+    1..16, depth 6; scene i uses seed seed0 + i."""
+    jobs = []
+    for i in range(n):
+        rng = np.random.default_rng(seed0 + i)
+        tris = int(math.exp(rng.uniform(math.log(min_tris), math.log(max_tris))))
+        w, h = BATCH_RESOLUTIONS[int(rng.integers(0, len(BATCH_RESOLUTIONS)))]
+        spp = int(rng.integers(1, 17))
+        jobs.append({"seed": seed0 + i, "target_tris": tris, "width": w, "height": h, "spp": spp,
+                     "max_depth": max_depth, "albedo": 0.8, "view": BATCH_VIEW})
+    return jobs
+
+
 def build_scene(name: str) -> Mesh:
     cfg = CONFIGS[name]
     if cfg["scene"] == "heightfield_box":
         return scene_heightfield_box(cfg["seed"])
     if cfg["scene"] == "spheres":
         return scene_spheres(cfg["seed"])
+    if cfg["scene"] == "heightfield_boxes":
+        return scene_heightfield_boxes(cfg["seed"])
     raise KeyError(name)

@@ -188,6 +188,84 @@ int prt_hip_accum_read(prt_hip_ctx* ctx, float* dst_rgba);
 /* Tonemapped 32-bit pixels b | g<<8 | r<<16 (accumBufferUpdateKernel). */
 int prt_hip_frame_read_bgr8(prt_hip_ctx* ctx, int32_t* dst);
 
+/* --- per-triangle RGB albedo and its adjoint (no counterpart in the
+ * reference, which hard-codes R = 0.8, render/diffuse_integrator_single.h:43) */
+/* Per-triangle RGB albedo, 3 floats per triangle in the order of the uploaded
+ * indices.  _host copies it; _device uses the caller's device array (not
+ * copied; it must stay alive and unchanged while the context renders). */
+int prt_hip_set_albedo_host(prt_hip_ctx* ctx, const float* rgb);
+int prt_hip_set_albedo_device(prt_hip_ctx* ctx, const float* d_rgb);
+/* Mean RGB image over passes pass0 .. pass0+spp-1 with the per-triangle
+ * albedo: a path that escapes after hitting t_0..t_{K-1} carries
+ * prod_k albedo(t_k) per channel, one hitting at max_depth carries 0.
+ * image: 3 floats per pixel, row-major.  Resets the accumulation buffer. */
+int prt_hip_render_albedo_image(prt_hip_ctx* ctx, int pass0, int spp, int max_depth, float* d_image,
+                                prt_render_stats* out_stats);
+int prt_hip_render_albedo_image_host(prt_hip_ctx* ctx, int pass0, int spp, int max_depth, float* image,
+                                     prt_render_stats* out_stats);
+/* Path-replay adjoint of prt_hip_render_albedo_image: grad += dLoss/dalbedo
+ * for dLoss/dImage = dldi (3 floats per pixel).  Each path is traced once to
+ * get its per-channel albedo product, then re-traced from its sampler seed
+ * (pass, pixelId) -- nothing about the path is stored -- and every hit
+ * atomically adds dldi * (product of the path's other albedos) / spp.
+ * rays in out_stats counts both traces. */
+int prt_hip_adjoint_albedo(prt_hip_ctx* ctx, int pass0, int spp, int max_depth, const float* d_dldi,
+                           float* d_grad, prt_render_stats* out_stats);
+int prt_hip_adjoint_albedo_host(prt_hip_ctx* ctx, int pass0, int spp, int max_depth, const float* dldi,
+                                float* grad, prt_render_stats* out_stats);
+
+/* --- batched form (no counterpart in the reference) ---------------------- */
+/* Many scenes, cameras and sample counts of uneven size in ONE persistent
+ * launch: a work item is (job, sample, path); waves steal work items from one
+ * counter across all jobs.  Each path writes its radiance to a per-sample
+ * plane and a finalize kernel adds the samples to the job's accumulator in
+ * pass order, so a job's image is bit-identical to rendering it with
+ * prt_hip_render one sample at a time.  The launch sequence (reset, batch
+ * kernel, finalize, pass advance) is captured once as a hipGraph and replayed
+ * by every prt_hip_batch_render; the bounce loop itself runs on the device
+ * inside the batch kernel.  The kernel's block size, LDS stack depth and
+ * in-flight path count come from prt_hip_batch_autotune. */
+typedef struct prt_batch_job {
+    int32_t scene;        /* index returned by prt_hip_batch_scene_add          */
+    int32_t width, height;
+    int32_t spp;          /* samples per pixel per prt_hip_batch_render call    */
+    int32_t max_depth;    /* 0..14                                              */
+    int32_t pass0;        /* sampler pass of the first sample                   */
+    float albedo;
+    prt_camera camera;    /* pinhole                                            */
+} prt_batch_job;
+
+typedef struct prt_batch_stats {
+    int64_t rays;         /* rays traced by the whole batch                      */
+    int64_t paths;        /* live paths = sum of spp * width * height            */
+    double ms;            /* device time of the batch render (graph launch)      */
+    double mray;
+    int64_t nodes, tris;  /* BVH8 nodes / triangle tests with PRT_PROFILE_COUNTERS */
+} prt_batch_stats;
+
+typedef struct prt_batch_tuning {
+    int32_t variant;       /* 0..3: (block, LDS stack) = (256,16) (256,8) (128,16) (128,8) */
+    int32_t block, stack;  /* filled in from the variant                          */
+    int32_t blocks_per_cu; /* resident workgroups per CU (paths in flight); 0 = occupancy maximum */
+    int32_t refill;        /* idle lanes of a wave that trigger a refill (1..64)  */
+    double ms;             /* autotuner: best measured time of this configuration */
+} prt_batch_tuning;
+
+int prt_hip_batch_scene_add(prt_hip_ctx* ctx, const int32_t* idx3, int64_t tri_count, const float* pos3,
+                            int64_t vertex_count, prt_build_stats* out_stats, int32_t* out_scene);
+int prt_hip_batch_reset(prt_hip_ctx* ctx);
+/* Allocates the accumulators (zeroed) and sample planes of the jobs. */
+int prt_hip_batch_set_jobs(prt_hip_ctx* ctx, const prt_batch_job* jobs, int n_jobs);
+int prt_hip_batch_render(prt_hip_ctx* ctx, prt_batch_stats* out_stats);
+int prt_hip_batch_accum_read(prt_hip_ctx* ctx, int job, float* dst_rgba);
+int prt_hip_batch_set_tuning(prt_hip_ctx* ctx, const prt_batch_tuning* tuning);
+int prt_hip_batch_get_tuning(prt_hip_ctx* ctx, prt_batch_tuning* tuning);
+/* Times every candidate launch shape on the current jobs (accumulators are
+ * left untouched), keeps the fastest and reports all of them. */
+int prt_hip_batch_autotune(prt_hip_ctx* ctx, prt_batch_tuning* results, int max_results, int* n_results);
+/* 1 (default): replay a captured hipGraph; 0: plain stream launches. */
+int prt_hip_batch_set_graph(prt_hip_ctx* ctx, int enable);
+
 /* --- inspection (tests and the roofline model) -------------------------- */
 /* Primary rays of a pass exactly as the diffuse renderer generates them,
  * one per pixel in row-major pixel order, plus their sampler states. */

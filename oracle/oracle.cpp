@@ -1007,6 +1007,122 @@ void orc_bounce(const float* in, int64_t n, int numerics, float* out)
     }
 }
 
+// ---------------------------------------------------------------- albedo
+// Path tape for the per-triangle albedo model: the geometry of a path does
+// not depend on the albedo, so each path (pixel, pass) is traced once with
+// the kernel numerics and its hit triangles are recorded.  The image, the
+// exact adjoint and finite differences are then evaluated in double over the
+// tape.  (The GPU adjoint re-traces paths instead of storing them.)
+struct PathTape {
+    int w, h, spp;
+    std::vector<int32_t> pixel;      // per path
+    std::vector<uint8_t> escaped;    // per path
+    std::vector<int64_t> start;      // per path + 1, into hits
+    std::vector<int32_t> hits;
+    int64_t rays = 0;
+};
+
+void* orc_paths_record(void* scp, const float* cam, int w, int h, int pass0, int spp, int max_depth, int threads)
+{
+    const Scene& sc = *static_cast<Scene*>(scp);
+    Cam c = cam_from(cam);
+    const float inv_w = 1.0f / (float)w, inv_h = 1.0f / (float)h;
+    const int64_t npx = (int64_t)w * h;
+    PathTape* tape = new PathTape();
+    tape->w = w;
+    tape->h = h;
+    tape->spp = spp;
+    const int64_t npaths = npx * spp;
+    std::vector<std::vector<int32_t>> path_hits(npaths);
+    tape->pixel.resize(npaths);
+    tape->escaped.resize(npaths);
+    std::atomic<int64_t> rays{0};
+    const int64_t chunk = 4096;
+    parallel_tiles((int)((npaths + chunk - 1) / chunk), default_threads(threads), [&](int tile) {
+        int64_t local = 0;
+        for (int64_t i = tile * chunk; i < std::min(npaths, (int64_t)(tile + 1) * chunk); ++i) {
+            const int sample = (int)(i / npx);
+            const int pid = (int)(i % npx);
+            const int x = pid % w, y = pid / w;
+            Sampler s{seed_of(pass0 + sample, pid)};
+            float u1 = s.next(), u2 = s.next();
+            s.next();
+            s.next();
+            V3 dir = cam_dir(c, ((float)x + u1) * inv_w, ((float)y + u2) * inv_h);
+            V3 org = c.origin;
+            std::vector<int32_t>& hl = path_hits[i];
+            int depth = 0;
+            HitAny hit;
+            for (;;) {
+                hit = closest_any(sc, org, dir, 0);
+                ++local;
+                if (hit.prim < 0 || depth == max_depth) break;
+                hl.push_back(hit.prim);
+                V3 n = facing_normal(sc.tris[hit.prim], dir);
+                float s1 = s.next(), s2 = s.next();
+                V3 no, nd;
+                bounce(org, dir, hit.t, n, s1, s2, 0, no, nd);
+                org = no;
+                dir = nd;
+                ++depth;
+            }
+            tape->pixel[i] = pid;
+            tape->escaped[i] = hit.prim < 0 ? 1 : 0;
+        }
+        rays += local;
+    });
+    tape->start.resize(npaths + 1);
+    tape->start[0] = 0;
+    for (int64_t i = 0; i < npaths; ++i) tape->start[i + 1] = tape->start[i] + (int64_t)path_hits[i].size();
+    tape->hits.resize(tape->start[npaths]);
+    for (int64_t i = 0; i < npaths; ++i)
+        std::copy(path_hits[i].begin(), path_hits[i].end(), tape->hits.begin() + tape->start[i]);
+    tape->rays = rays.load();
+    return tape;
+}
+
+void orc_paths_destroy(void* t) { delete static_cast<PathTape*>(t); }
+
+int64_t orc_paths_rays(void* t) { return static_cast<PathTape*>(t)->rays; }
+
+// mean RGB image over the tape's samples; albedo: 3 doubles per triangle
+void orc_paths_image(void* tp, const double* albedo, double* image)
+{
+    const PathTape& t = *static_cast<PathTape*>(tp);
+    const int64_t npx = (int64_t)t.w * t.h;
+    std::fill(image, image + 3 * npx, 0.0);
+    const int64_t npaths = (int64_t)t.pixel.size();
+    for (int64_t i = 0; i < npaths; ++i) {
+        if (!t.escaped[i]) continue;
+        double l[3] = {1.0, 1.0, 1.0};
+        for (int64_t k = t.start[i]; k < t.start[i + 1]; ++k)
+            for (int c = 0; c < 3; ++c) l[c] *= albedo[3 * (size_t)t.hits[k] + c];
+        for (int c = 0; c < 3; ++c) image[3 * (size_t)t.pixel[i] + c] += l[c];
+    }
+    for (int64_t i = 0; i < 3 * npx; ++i) image[i] /= (double)t.spp;
+}
+
+// grad (3 doubles per triangle) += dLoss/dalbedo for dLoss/dImage = dldi
+void orc_paths_adjoint(void* tp, const double* albedo, const double* dldi, double* grad)
+{
+    const PathTape& t = *static_cast<PathTape*>(tp);
+    const int64_t npaths = (int64_t)t.pixel.size();
+    for (int64_t i = 0; i < npaths; ++i) {
+        if (!t.escaped[i]) continue;
+        const int64_t b = t.start[i], e = t.start[i + 1];
+        for (int c = 0; c < 3; ++c) {
+            const double wgt = dldi[3 * (size_t)t.pixel[i] + c] / (double)t.spp;
+            if (wgt == 0.0) continue;
+            for (int64_t k = b; k < e; ++k) {
+                double others = 1.0;  // product of the other hits' albedos
+                for (int64_t j = b; j < e; ++j)
+                    if (j != k) others *= albedo[3 * (size_t)t.hits[j] + c];
+                grad[3 * (size_t)t.hits[k] + c] += wgt * others;
+            }
+        }
+    }
+}
+
 int orc_thread_count(void) { return default_threads(0); }
 
 }  // extern "C"

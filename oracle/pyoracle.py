@@ -47,6 +47,13 @@ def lib() -> ctypes.CDLL:
         L.orc_stream_bench.restype = i
         L.orc_bounce.argtypes = [P, i64, i, P]
         L.orc_thread_count.restype = i
+        L.orc_paths_record.argtypes = [P, P, i, i, i, i, i, i]
+        L.orc_paths_record.restype = P
+        L.orc_paths_destroy.argtypes = [P]
+        L.orc_paths_rays.argtypes = [P]
+        L.orc_paths_rays.restype = i64
+        L.orc_paths_image.argtypes = [P, P, P]
+        L.orc_paths_adjoint.argtypes = [P, P, P, P]
         _LIB = L
     return _LIB
 
@@ -155,6 +162,45 @@ class Scene:
         lib().orc_stream_bench(self.h, _p(cam), w, h, row0, row1, pass0, frames, max_depth, albedo, threads,
                                _p(accum), ctypes.byref(rays), ctypes.byref(secs))
         return {"accum": accum, "rays": rays.value, "seconds": secs.value}
+
+
+class PathTape:
+    """Hit lists of every path (pixel, pass) of a render, traced once with the
+    kernel numerics; the per-triangle albedo model is then evaluated in double."""
+
+    def __init__(self, scene: "Scene", cam: np.ndarray, w: int, h: int, pass0: int, spp: int, max_depth: int,
+                 threads: int = 0):
+        cam = np.ascontiguousarray(cam, dtype=np.float32)
+        self.w, self.h, self.tri_count = w, h, scene.idx.shape[0]
+        self.h_ = lib().orc_paths_record(scene.h, _p(cam), w, h, pass0, spp, max_depth, threads)
+
+    def close(self) -> None:
+        if self.h_:
+            lib().orc_paths_destroy(self.h_)
+            self.h_ = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def rays(self) -> int:
+        return int(lib().orc_paths_rays(self.h_))
+
+    def image(self, albedo: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(albedo, dtype=np.float64).reshape(-1)
+        out = np.zeros((self.h, self.w, 3), dtype=np.float64)
+        lib().orc_paths_image(self.h_, _p(a), _p(out))
+        return out
+
+    def adjoint(self, albedo: np.ndarray, dldi: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(albedo, dtype=np.float64).reshape(-1)
+        g = np.ascontiguousarray(dldi, dtype=np.float64).reshape(-1)
+        out = np.zeros((self.tri_count, 3), dtype=np.float64)
+        lib().orc_paths_adjoint(self.h_, _p(a), _p(g), _p(out))
+        return out
 
 
 def bounce(inp: np.ndarray, numerics: int = 0) -> np.ndarray:
