@@ -129,6 +129,89 @@ def bench_batched(pkg, scenes, dev, mesh, cam, cfg, args, plain_mrays: float, wa
             "autotune_candidates": len(tune3)}
 
 
+def bench_adjoint(pkg, scenes, args) -> dict:
+    """Differentiable workload: 128x128 heightfield + 64 boxes (33,536
+    triangles), 512x512, 8 spp, depth 6, per-triangle RGB albedo U[0.1, 0.9];
+    L2 loss against a target rendered from a second albedo set.  Times the
+    forward image (8 one-sample launches) and the path-replay adjoint (one
+    launch over all 8 samples)."""
+    cfg = scenes.CONFIGS["heightfield_boxes_512"]
+    w, h, spp, depth = cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"]
+    mesh = scenes.build_scene("heightfield_boxes_512")
+    v = cfg["view"]
+    cam = pkg.camera_from_view(v["pos"], v["angle_x"], v["angle_y"], v["fov_y"], w / h)
+    a = scenes.albedo_set(mesh.tri_count, cfg["albedo_seed"])
+    b = scenes.albedo_set(mesh.tri_count, cfg["target_seed"])
+    with pkg.HipDevice(0) as dev:
+        dev.upload_scene(mesh.idx, mesh.pos)
+        dev.build()
+        dev.set_camera(cam)
+        dev.init_frame(w, h)
+        dev.set_albedo(b)
+        target, _ = dev.render_albedo_image(0, spp, depth)
+        dev.set_albedo(a)
+        img, _ = dev.render_albedo_image(0, spp, depth)
+        dldi = 2.0 * (img - target)
+        dev.adjoint_albedo(0, spp, depth, dldi)  # warm-up
+        reps = 5
+        fwd = [dev.render_albedo_image(0, spp, depth)[1] for _ in range(reps)]
+        adj = [dev.adjoint_albedo(0, spp, depth, dldi)[1] for _ in range(reps)]
+    f_ms = min(s["renderMs"] for s in fwd)
+    a_ms = min(s["renderMs"] for s in adj)
+    a_rays = adj[0]["ray"]
+    out = {"workload": "heightfield_boxes_512: 33,536 tris, 512x512, 8 spp, depth 6, per-triangle RGB albedo",
+           "forward_ms": round(f_ms, 3), "forward_mrays": round(fwd[0]["ray"] / f_ms / 1e3, 2),
+           "adjoint_ms": round(a_ms, 3), "adjoint_mrays": round(a_rays / a_ms / 1e3, 2),
+           "adjoint_over_forward": round(a_ms / f_ms, 3), "adjoint_rays": a_rays, "forward_rays": fwd[0]["ray"],
+           "loss": float(((img.astype(np.float64) - target) ** 2).sum())}
+    log(f"adjoint: forward {f_ms:.2f} ms, adjoint {a_ms:.2f} ms ({a_ms / f_ms:.2f}x), "
+        f"{out['adjoint_mrays']} Mrays/s")
+    return out
+
+
+def bench_city(pkg, scenes, args) -> dict:
+    """Large stress workload: 1.33 M boxes (15.96 M triangles), 3840x2160,
+    max depth 8: LBVH->BVH8 build rate, the batched forward (all spp in one
+    launch) and the per-triangle albedo adjoint (16 M x 3 fp32 gradient)."""
+    cfg = scenes.CONFIGS["city_4k"]
+    w, h, depth = cfg["width"], cfg["height"], cfg["max_depth"]
+    spp = args.city_spp
+    mesh = scenes.build_scene("city_4k")
+    v = cfg["view"]
+    cam = pkg.camera_from_view(v["pos"], v["angle_x"], v["angle_y"], v["fov_y"], w / h)
+    out = {"workload": f"city_4k: {mesh.tri_count} tris, {w}x{h}, {spp} spp, depth {depth}", "spp": spp}
+    with pkg.HipDevice(0) as dev:
+        sid, bst = dev.batch_add_scene(mesh.idx, mesh.pos)
+        out["buildMs"] = round(bst["buildMs"], 2)
+        out["buildMprim"] = round(bst["buildMprim"], 1)
+        out["bvh_depth"] = bst["depth"]
+        job = {"scene": sid, "width": w, "height": h, "spp": spp, "max_depth": depth, "albedo": cfg["albedo"],
+               "camera": cam}
+        dev.batch_set_jobs([job])
+        dev.batch_render()
+        st = dev.batch_render()
+        out["batched_mrays"] = round(st["ray"] / st["renderMs"] / 1e3, 2)
+        out["batched_ms"] = round(st["renderMs"], 2)
+        out["rays_per_launch"] = st["ray"]
+        log(f"city: build {bst['buildMs']:.1f} ms ({bst['buildMprim']:.0f} Mprim/s), batched forward "
+            f"{out['batched_mrays']} Mrays/s ({st['ray']} rays)")
+        dev.batch_reset()
+        dev.upload_scene(mesh.idx, mesh.pos)
+        dev.build()
+        dev.set_camera(cam)
+        dev.init_frame(w, h)
+        dev.set_albedo(scenes.albedo_set(mesh.tri_count, cfg["albedo_seed"]))
+        aspp = args.city_adjoint_spp
+        img, fst = dev.render_albedo_image(0, aspp, depth)
+        grad, ast = dev.adjoint_albedo(0, aspp, depth, img)  # loss = 0.5 * sum(image^2)
+        out.update({"adjoint_spp": aspp, "forward_ms": round(fst["renderMs"], 2),
+                    "adjoint_ms": round(ast["renderMs"], 2), "adjoint_mrays": round(ast["ray"] / ast["renderMs"] / 1e3, 2),
+                    "adjoint_over_forward": round(ast["renderMs"] / fst["renderMs"], 3),
+                    "grad_nonzero_triangles": int((np.abs(grad).sum(1) > 0).sum())})
+        log(f"city adjoint ({aspp} spp): forward {fst['renderMs']:.1f} ms, adjoint {ast['renderMs']:.1f} ms")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +220,10 @@ def main() -> int:
     ap.add_argument("--cpu-frames", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-city", action="store_true")
+    ap.add_argument("--city-spp", type=int, default=64)
+    ap.add_argument("--city-adjoint-spp", type=int, default=8)
     ap.add_argument("--batch-scenes", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -239,6 +326,10 @@ def main() -> int:
         "cpu_baseline": None,
     }
     dev.close()
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint(pkg, scenes, args)
+    if not args.no_city:
+        out["city"] = bench_city(pkg, scenes, args)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(scenes, mesh, cfg, cam, args.cpu_frames)

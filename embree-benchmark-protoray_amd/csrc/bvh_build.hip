@@ -7,7 +7,8 @@
 //   3. LSD radix sort of (key, triangle id), 8 passes of 8 bits
 //   4. radix tree (T. Karras, "Maximizing Parallelism in the Construction of
 //      BVHs, Octrees, and k-d Trees", HPG 2012), equal keys split by index
-//   5. bottom-up boxes with one atomic arrival counter per internal node
+//   5. bottom-up boxes with one arrival counter per internal node: inside
+//      4096-leaf tiles in LDS, across tiles with device-scope atomics
 //   6. top-down collapse to BVH8 one level per launch: the widest-area
 //      internal child is opened until 8 slots are used; subtrees of at most
 //      kLeafMax triangles become leaf slots.  Node and record offsets come
@@ -314,24 +315,85 @@ __device__ __forceinline__ Box child_box(int32_t c, const uint32_t* sorted, cons
     return b;
 }
 
-__global__ __launch_bounds__(kBlock) void k_refit(int64_t n, const uint32_t* sorted, const int32_t* idx,
-                                                  const float* pos, const int32_t* left,
-                                                  const int32_t* right, const int32_t* iparent,
-                                                  const int32_t* lparent, float* ibox, uint32_t* flags)
+// Two-phase refit.  Phase A: one workgroup per tile of kRefitTile sorted
+// leaves finishes every internal node whose leaf range lies inside the tile.
+// A Karras node's index is one end of its range, so those nodes have indices
+// inside the tile too and their arrival flags and boxes live in LDS; only
+// workgroup-scope ordering is needed.  A walk that reaches a node crossing
+// the tile border appends it to `pending`.  Phase B runs the device-scope
+// arrival protocol above for the crossing nodes only.
+constexpr int kRefitTile = 4096;
+constexpr int kRefitThreads = 1024;
+
+__global__ __launch_bounds__(kRefitThreads) void k_refit_local(
+    int64_t n, const uint32_t* sorted, const int32_t* idx, const float* pos, const int32_t* left,
+    const int32_t* right, const int32_t* first, const int32_t* last, const int32_t* iparent,
+    const int32_t* lparent, float* ibox, int32_t* pending, uint32_t* pending_count)
 {
-    int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (j >= n) return;
-    int32_t p = lparent[j];
+    __shared__ uint32_t flag[kRefitTile];
+    __shared__ float lbox[kRefitTile][6];
+    const int64_t lo = (int64_t)blockIdx.x * kRefitTile;
+    const int64_t hi = (lo + kRefitTile < n ? lo + kRefitTile : n) - 1;
+    for (int i = threadIdx.x; i < kRefitTile; i += kRefitThreads) flag[i] = 0u;
+    __syncthreads();
+    for (int k = 0; k < kRefitTile / kRefitThreads; ++k) {
+        const int64_t j = lo + (int64_t)k * kRefitThreads + threadIdx.x;
+        if (j > hi) break;
+        Box b = tri_box(idx, pos, (int)sorted[j]);
+        int32_t cur = ~(int32_t)j;
+        int32_t p = lparent[j];
+        while (p >= 0) {
+            if (first[p] < lo || last[p] > hi) {
+                const uint32_t slot = atomicAdd(pending_count, 1u);
+                pending[slot] = p;
+                break;
+            }
+            const int s = (int)(p - lo);
+            __threadfence_block();
+            if (atomicAdd(&flag[s], 1u) == 0u) break;
+            __threadfence_block();
+            const int32_t sib = left[p] == cur ? right[p] : left[p];
+            Box o;
+            if (sib < 0) {
+                o = tri_box(idx, pos, (int)sorted[~sib]);
+            } else {
+                for (int a = 0; a < 3; ++a) {
+                    o.lo[a] = lbox[sib - lo][a];
+                    o.hi[a] = lbox[sib - lo][3 + a];
+                }
+            }
+            for (int a = 0; a < 3; ++a) {
+                b.lo[a] = fminf(b.lo[a], o.lo[a]);
+                b.hi[a] = fmaxf(b.hi[a], o.hi[a]);
+                lbox[s][a] = b.lo[a];
+                lbox[s][3 + a] = b.hi[a];
+                ibox[6 * (int64_t)p + a] = b.lo[a];
+                ibox[6 * (int64_t)p + 3 + a] = b.hi[a];
+            }
+            cur = p;
+            p = iparent[p];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit_crossing(const int32_t* pending, const uint32_t* pending_count,
+                                                           const uint32_t* sorted, const int32_t* idx,
+                                                           const float* pos, const int32_t* left,
+                                                           const int32_t* right, const int32_t* iparent,
+                                                           float* ibox, uint32_t* flags)
+{
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= *pending_count) return;
+    int32_t p = pending[k];
     while (p >= 0) {
         __threadfence();
-        uint32_t old = atomicAdd(&flags[p], 1u);
-        if (old == 0) return;
+        if (atomicAdd(&flags[p], 1u) == 0u) return;
         __threadfence();
         Box a = child_box(left[p], sorted, idx, pos, ibox);
         Box b = child_box(right[p], sorted, idx, pos, ibox);
-        for (int k = 0; k < 3; ++k) {
-            ibox[6 * (int64_t)p + k] = fminf(a.lo[k], b.lo[k]);
-            ibox[6 * (int64_t)p + 3 + k] = fmaxf(a.hi[k], b.hi[k]);
+        for (int c = 0; c < 3; ++c) {
+            ibox[6 * (int64_t)p + c] = fminf(a.lo[c], b.lo[c]);
+            ibox[6 * (int64_t)p + 3 + c] = fmaxf(a.hi[c], b.hi[c]);
         }
         p = iparent[p];
     }
@@ -567,7 +629,9 @@ int build_bvh8(prt_hip_ctx* ctx, SceneBuffers& sb, prt_build_stats* stats)
     PRT_TRY(ctx, dev_alloc(iparent, 4 * ni));
     PRT_TRY(ctx, dev_alloc(lparent, 4 * n));
     PRT_TRY(ctx, dev_alloc(ibox, 4 * 6 * ni));
-    PRT_TRY(ctx, dev_alloc(flags, 4 * ni));
+    PRT_TRY(ctx, dev_alloc(flags, 4 * (ni + 1)));
+    DevBuf pending;
+    PRT_TRY(ctx, dev_alloc(pending, 4 * 2 * ni));
     PRT_TRY(ctx, dev_alloc(tasks[0], 4 * ni));
     PRT_TRY(ctx, dev_alloc(tasks[1], 4 * ni));
     PRT_TRY(ctx, dev_alloc(slots, 4 * 8 * ni));
@@ -646,11 +710,19 @@ int build_bvh8(prt_hip_ctx* ctx, SceneBuffers& sb, prt_build_stats* stats)
         k_radix_tree<<<blocks_for(n - 1, kBlock), kBlock, 0, s>>>(
             sk, n, left.as<int32_t>(), right.as<int32_t>(), first.as<int32_t>(), last.as<int32_t>(),
             iparent.as<int32_t>(), lparent.as<int32_t>());
-        PRT_TRY(ctx, hipMemsetAsync(flags.p, 0, 4 * ni, s));
-        k_refit<<<blocks_for(n, kBlock), kBlock, 0, s>>>(n, sv, sb.d_idx, sb.d_pos, left.as<int32_t>(),
-                                                          right.as<int32_t>(), iparent.as<int32_t>(),
-                                                          lparent.as<int32_t>(), ibox.as<float>(),
-                                                          flags.as<uint32_t>());
+        PRT_TRY(ctx, hipMemsetAsync(flags.p, 0, 4 * (ni + 1), s));
+        uint32_t* pending_count = flags.as<uint32_t>() + ni;
+        const int tiles = blocks_for(n, kRefitTile);
+        k_refit_local<<<tiles, kRefitThreads, 0, s>>>(n, sv, sb.d_idx, sb.d_pos, left.as<int32_t>(),
+                                                      right.as<int32_t>(), first.as<int32_t>(), last.as<int32_t>(),
+                                                      iparent.as<int32_t>(), lparent.as<int32_t>(), ibox.as<float>(),
+                                                      pending.as<int32_t>(), pending_count);
+        PRT_TRY(ctx, hipGetLastError());
+        // at most two arrivals per crossing node; bounded by the internal node count
+        k_refit_crossing<<<blocks_for(2 * ni, kBlock), kBlock, 0, s>>>(pending.as<int32_t>(), pending_count, sv,
+                                                                   sb.d_idx, sb.d_pos, left.as<int32_t>(),
+                                                                   right.as<int32_t>(), iparent.as<int32_t>(),
+                                                                   ibox.as<float>(), flags.as<uint32_t>());
         PRT_TRY(ctx, hipGetLastError());
 
         // collapse, one level per iteration

@@ -403,7 +403,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             const uint32_t fz0 = neg_z ? w3.x : w4.z, fz1 = neg_z ? w3.y : w4.w;
             const uint32_t meta0 = w1.z, meta1 = w1.w;
 
-            uint32_t hit_int = 0, hit_leaf = 0;
+            // branchless child tests; empty slots carry inverted boxes, and a
+            // leaf slot with meta 0 holds no triangle, so no leaf mask is needed
+            uint32_t hits = 0;
             float tk[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
@@ -420,12 +422,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                 const float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
                 const float tf = fminf(fminf(fminf(tfx, tfy), tfz), tbest);
                 tk[k] = tn;
-                const uint32_t mb = ((k < 4 ? meta0 : meta1) >> (8 * kb)) & 0xffu;
-                if (tn <= tf) {
-                    if ((imask >> k) & 1u) hit_int |= 1u << k;
-                    else if (mb != 0u) hit_leaf |= 1u << k;
-                }
+                hits |= (tn <= tf ? 1u : 0u) << k;
             }
+            uint32_t hit_int = hits & imask;
+            uint32_t hit_leaf = hits & ~imask;
 
             // ---- leaf slots: test their triangles
             const uint32_t tri_base = w1.y;

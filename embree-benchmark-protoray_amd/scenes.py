@@ -257,6 +257,11 @@ CONFIGS = {
         "view": {"pos": [0.0, 1.9, 2.1], "angle_x": 0.75, "angle_y": 0.0, "fov_y": math.radians(50.0)},
         "width": 512, "height": 512, "spp": 8, "max_depth": 6, "albedo": None,
     },
+    "city_4k": {
+        "scene": "city", "seed": 1005, "albedo_seed": 3005,
+        "view": {"pos": [0.5, 6.0, 2.6], "angle_x": 1.17, "angle_y": 0.0, "fov_y": math.radians(40.0)},
+        "width": 3840, "height": 2160, "spp": 64, "max_depth": 8, "albedo": 0.8,
+    },
     "spheres_1080p": {
         "scene": "spheres", "seed": 1002,
         "view": {"pos": [0.5, 0.9, 2.6], "angle_x": 0.25, "angle_y": 0.0, "fov_y": math.radians(45.0)},
@@ -297,6 +302,27 @@ def scene_heightfield_boxes(seed: int = 1004, cells: int = 128, boxes: int = 64)
     return merge(parts)
 
 
+def scene_city(seed: int = 1005, boxes: int = 1_330_000) -> Mesh:
+    """Large stress workload: `boxes` closed axis-aligned boxes (12 triangles,
+    8 vertices each; 1.33 M boxes = 15.96 M triangles) with footprints
+    U[0.002, 0.01] per axis, centres U[0,1]^2 on the ground y = 0, and heights
+    log-normal(mu=0, sigma=1): a dense field of thin towers.  Overlapping
+    boxes produce coincident faces, which exercise the (t, primId) tie rule."""
+    rng = np.random.default_rng(seed)
+    cx = rng.uniform(0.0, 1.0, boxes)
+    cz = rng.uniform(0.0, 1.0, boxes)
+    sx = rng.uniform(0.002, 0.01, boxes)
+    sz = rng.uniform(0.002, 0.01, boxes)
+    hy = np.exp(rng.normal(0.0, 1.0, boxes))
+    lo = np.stack([cx - sx / 2, np.zeros(boxes), cz - sz / 2], 1).astype(np.float32)
+    hi = np.stack([cx + sx / 2, hy, cz + sz / 2], 1).astype(np.float32)
+    bi, bp = box_tris((0, 0, 0), (1, 1, 1))          # unit-box topology
+    sel = bp.astype(bool)                            # which corner takes hi per axis
+    pos = np.where(sel[None, :, :], hi[:, None, :], lo[:, None, :]).reshape(-1, 3)
+    idx = (bi[None, :, :] + 8 * np.arange(boxes, dtype=np.int64)[:, None, None]).reshape(-1, 3)
+    return Mesh(idx.astype(np.int32), np.ascontiguousarray(pos, dtype=np.float32))
+
+
 def albedo_set(tri_count: int, seed: int, lo: float = 0.1, hi: float = 0.9) -> np.ndarray:
     """Per-triangle RGB albedo, uniform in [lo, hi], shape (tri_count, 3)."""
     return np.random.default_rng(seed).uniform(lo, hi, size=(tri_count, 3)).astype(np.float32)
@@ -330,4 +356,6 @@ def build_scene(name: str) -> Mesh:
         return scene_spheres(cfg["seed"])
     if cfg["scene"] == "heightfield_boxes":
         return scene_heightfield_boxes(cfg["seed"])
+    if cfg["scene"] == "city":
+        return scene_city(cfg["seed"])
     raise KeyError(name)
