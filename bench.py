@@ -10,13 +10,20 @@ ray handed to the closest-hit query, primary and secondary, divided by the
 timed seconds (render/diffuse_renderer_cuda.cu:186).
 
 Prints ONE JSON line.  `value` is wall-clock Mrays/s over the K timed frames
-with the scene, BVH and queues resident in HBM.  `roofline` is for the
-dominant kernel (the closest-hit traversal): algorithmic bytes per ray
-48 + 80 * nodes/ray + 36 * tris/ray (ray in, hit out, BVH8 nodes and triangle
-records fetched), nodes/tris per ray from an instrumented replay of the same
-frames, divided by the traversal launches' device time (HIP events on the
-library's stream).  `cpu_baseline` is the oracle's restatement of the CPU
-stream renderer (render/diffuse_renderer_stream.h) on the host cores.
+with the scene, BVH and path state resident in HBM.  The default pipeline is
+the fused persistent path kernel (raygen, closest hit, cosine bounce and
+accumulation in one launch per frame), which is the dominant and only kernel
+of a frame.  `roofline` is for it: algorithmic bytes per frame
+rays * (80 * nodes/ray + 36 * tris/ray) + 32 * paths (BVH8 nodes and triangle
+records the traversal fetches, the accumulator read + write per path), with
+nodes/tris per ray from an instrumented replay of the same frames, divided by
+the kernel's device time (HIP events on the library's stream).  The
+per-depth wavefront pipeline is timed beside it and its algorithmic bytes
+(ray and hit queues in HBM) are reported too.  `cpu_baseline` is the oracle's
+restatement of the CPU stream renderer (render/diffuse_renderer_stream.h) on
+the host cores.  Further legs: the batched form (one persistent launch over
+many scenes, cameras and spp), the per-triangle albedo adjoint, and the
+15.96M-triangle city scene.
 """
 from __future__ import annotations
 
