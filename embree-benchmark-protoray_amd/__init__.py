@@ -10,6 +10,7 @@ constructing HipDevice fails.
 from __future__ import annotations
 
 import ctypes
+import importlib.util
 import os
 import re
 
@@ -22,7 +23,8 @@ HEADER_PATH = os.path.join(REPO_DIR, "include", "prt_hip.h")
 
 RENDERER_PRIMARY = 0
 RENDERER_DIFFUSE = 1
-RENDERER_IDS = {"primaryHip": RENDERER_PRIMARY, "diffuseHip": RENDERER_DIFFUSE}
+RENDERER_DIFFUSE2 = 2
+RENDERER_IDS = {"primaryHip": RENDERER_PRIMARY, "diffuseHip": RENDERER_DIFFUSE, "diffuse2Hip": RENDERER_DIFFUSE2}
 
 RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tnear", "<f4"), ("dir", "<f4", 3), ("tfar", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_id", "<i4"), ("u", "<f4"), ("v", "<f4")])
@@ -81,6 +83,17 @@ def declared_symbols(header: str = HEADER_PATH) -> list[str]:
 _LIB = None
 
 
+def share_torch_runtime() -> None:
+    """One HIP runtime per process.  PyTorch's ROCm wheel carries its own
+    libamdhip64 (same soname as /opt/rocm's); a process that loads both cannot
+    use the second one -- torch reports no GPU if the library came first.  When
+    torch is installed it is therefore imported before the library, which then
+    binds to the runtime torch already holds, so device pointers and streams
+    are shared with torch tensors."""
+    if importlib.util.find_spec("torch") is not None:
+        import torch  # noqa: F401  (loads torch's HIP runtime)
+
+
 def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     """Load libprt_hip.so (built in-tree by `make -C embree-benchmark-protoray_amd`)."""
     global _LIB
@@ -88,6 +101,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         return _LIB
     if not os.path.exists(path):
         raise RuntimeError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    share_torch_runtime()
     lib = ctypes.CDLL(path)
     P = ctypes.c_void_p
     i64 = ctypes.c_int64
@@ -102,6 +116,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "prt_hip_intersect_closest": ([P, P, P, i64], ctypes.c_int),
         "prt_hip_intersect_closest_device": ([P, P, P, i64], ctypes.c_int),
         "prt_hip_intersect_counters": ([P, P, P, i64, ctypes.POINTER(PrtTraceCounters)], ctypes.c_int),
+        "prt_hip_occluded": ([P, P, P, i64], ctypes.c_int),
+        "prt_hip_occluded_device": ([P, P, P, i64], ctypes.c_int),
         "prt_hip_camera_from_view": ([ctypes.POINTER(ctypes.c_float * 3), ctypes.c_float, ctypes.c_float,
                                       ctypes.c_float, ctypes.c_float, ctypes.POINTER(PrtCamera)], ctypes.c_int),
         "prt_hip_camera_set": ([P, ctypes.POINTER(PrtCamera)], ctypes.c_int),
@@ -252,6 +268,14 @@ class HipDevice:
         self._check(self.lib.prt_hip_intersect_closest(self.ctx, _ptr(rays), _ptr(hits), rays.shape[0]),
                     "intersect_closest")
         return hits
+
+    def occluded(self, rays: np.ndarray) -> np.ndarray:
+        """Any-hit query (IntersectorStreamCuda::occluded): per ray, the distance
+        of some blocking triangle in [tnear, tfar], or -1."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        out = np.zeros(rays.shape[0], dtype=np.float32)
+        self._check(self.lib.prt_hip_occluded(self.ctx, _ptr(rays), _ptr(out), rays.shape[0]), "occluded")
+        return out
 
     def intersect_counters(self, rays: np.ndarray) -> tuple[np.ndarray, dict]:
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)

@@ -131,6 +131,22 @@ int intersect_impl(prt_hip_ctx* ctx, const prt_ray* d_rays, prt_hit* d_hits, int
     return 0;
 }
 
+int occluded_impl(prt_hip_ctx* ctx, const prt_ray* d_rays, float* d_t, int64_t n)
+{
+    if (!ctx->scene.bvh_ready) return set_error(ctx, "occluded: no BVH (call prt_hip_bvh_build)");
+    if (n < 0 || n > (int64_t)UINT32_MAX / 2) return set_error(ctx, "occluded: bad ray count");
+    if (ensure_trace_resources(ctx)) return 1;
+    PRT_TRY(ctx, launch_zero_u32(ctx->d_counters, 33, ctx->stream));
+    TraceLaunch t{};
+    t.rays = reinterpret_cast<const float4*>(d_rays);
+    t.any_t = d_t;
+    t.count_ptr = nullptr;
+    t.count = (uint32_t)n;
+    t.fetch_ptr = ctx->d_counters + 16;
+    PRT_TRY(ctx, launch_occluded(ctx, t, false));
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -256,6 +272,27 @@ int prt_hip_intersect_closest(prt_hip_ctx* ctx, const prt_ray* rays, prt_hit* hi
                        reinterpret_cast<prt_hit*>(ctx->d_io_hits), n, nullptr))
         return 1;
     PRT_TRY(ctx, hipMemcpyAsync(hits, ctx->d_io_hits, sizeof(prt_hit) * n, hipMemcpyDeviceToHost, ctx->stream));
+    PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return check_trace_error(ctx);
+}
+
+int prt_hip_occluded_device(prt_hip_ctx* ctx, const prt_ray* d_rays, float* d_out_t, int64_t n)
+{
+    if (use_device(ctx)) return 1;
+    if (n > 0 && (!d_rays || !d_out_t)) return set_error(ctx, "occluded: null array");
+    return occluded_impl(ctx, d_rays, d_out_t, n);
+}
+
+int prt_hip_occluded(prt_hip_ctx* ctx, const prt_ray* rays, float* out_t, int64_t n)
+{
+    if (use_device(ctx)) return 1;
+    if (n == 0) return 0;
+    if (!rays || !out_t) return set_error(ctx, "occluded: null array");
+    if (ensure_io(ctx, n)) return 1;
+    float* d_t = reinterpret_cast<float*>(ctx->d_io_hits);  // n floats fit in the n-hit buffer
+    PRT_TRY(ctx, hipMemcpyAsync(ctx->d_io_rays, rays, sizeof(prt_ray) * n, hipMemcpyHostToDevice, ctx->stream));
+    if (occluded_impl(ctx, reinterpret_cast<prt_ray*>(ctx->d_io_rays), d_t, n)) return 1;
+    PRT_TRY(ctx, hipMemcpyAsync(out_t, d_t, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
     PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return check_trace_error(ctx);
 }
@@ -410,8 +447,10 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
     if (!ctx->scene.bvh_ready) return set_error(ctx, "render: no BVH");
     if (!ctx->cam_ready) return set_error(ctx, "render: no camera");
     if (!ctx->d_accum) return set_error(ctx, "render: no frame");
-    if (renderer != PRT_RENDERER_PRIMARY && renderer != PRT_RENDERER_DIFFUSE)
+    if (renderer != PRT_RENDERER_PRIMARY && renderer != PRT_RENDERER_DIFFUSE && renderer != PRT_RENDERER_DIFFUSE2)
         return set_error(ctx, "render: invalid renderer type");
+    if (renderer == PRT_RENDERER_DIFFUSE2 && ctx->pipeline != PRT_PIPELINE_FUSED)
+        return set_error(ctx, "render: diffuse2 runs on the fused pipeline only");
     if (max_depth < 0 || max_depth + 1 >= kMaxQueueDepth)
         return set_error(ctx, "render: max_depth must be in [0, 14]");
     if (ensure_trace_resources(ctx)) return 1;
@@ -442,7 +481,7 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
         p.depth_rays = ctx->d_stats + 16;
         p.stats = ctx->d_stats;
         if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[0], ctx->stream));
-        PRT_TRY(ctx, launch_paths(ctx, p, renderer == PRT_RENDERER_PRIMARY, counting));
+        PRT_TRY(ctx, launch_paths(ctx, p, renderer, counting));
         if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[1], ctx->stream));
     } else {
         PRT_TRY(ctx, launch_raygen(ctx, ctx->pass, ctx->d_rays[0], ctx->d_pix[0], ctx->d_rng[0]));

@@ -52,6 +52,7 @@ void DeviceHip::initRenderer(const Props& props, Props& stats)
     const std::string type = props.get<std::string>("type");
     if (type == "primaryHip" || type == "primary") renderer_ = PRT_RENDERER_PRIMARY;
     else if (type == "diffuseHip" || type == "diffuse") renderer_ = PRT_RENDERER_DIFFUSE;
+    else if (type == "diffuse2Hip" || type == "diffuse2") renderer_ = PRT_RENDERER_DIFFUSE2;
     else throw std::invalid_argument("invalid renderer type");
 
     std::string sampler = props.get("sampler", "default");

@@ -275,7 +275,7 @@ int render_main(const std::vector<std::string>& args)
 int main(int argc, char* argv[])
 {
     if (argc < 2 || std::string(argv[1]) != "render") {
-        std::cout << "Usage: protoray render <scene.mesh> [-dev hip] [-r primaryHip|diffuseHip] [-size W,H] "
+        std::cout << "Usage: protoray render <scene.mesh> [-dev hip] [-r primaryHip|diffuseHip|diffuse2Hip] [-size W,H] "
                      "[-spp N] [-maxDepth D] [-albedo R] [-view N] [-benchmark prefix]"
                   << std::endl;
         return argc < 2 ? 0 : 1;

@@ -172,7 +172,8 @@ int build_bvh8(prt_hip_ctx* ctx, SceneBuffers& sb, prt_build_stats* stats);
 // trace.hip
 struct TraceLaunch {
     const float4* rays;
-    float4* hits;
+    float4* hits;               // closest-hit records
+    float* any_t;               // any-hit query: distance of a blocking triangle, or -1
     const uint32_t* count_ptr;  // device count of rays (nullptr: use count)
     uint32_t count;
     uint32_t* fetch_ptr;        // device fetch counter, zero before launch
@@ -201,7 +202,9 @@ struct PathLaunch {
 };
 int ensure_trace_resources(prt_hip_ctx* ctx);
 hipError_t launch_trace(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumented);
-hipError_t launch_paths(prt_hip_ctx* ctx, const PathLaunch& p, bool primary, bool instrumented);
+hipError_t launch_occluded(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumented);
+// renderer: PRT_RENDERER_PRIMARY, _DIFFUSE or _DIFFUSE2
+hipError_t launch_paths(prt_hip_ctx* ctx, const PathLaunch& p, int renderer, bool instrumented);
 hipError_t launch_paths_rgb(prt_hip_ctx* ctx, const PathLaunch& p, bool instrumented);
 hipError_t launch_adjoint(prt_hip_ctx* ctx, const PathLaunch& p, bool instrumented);
 // max_blocks_per_cu != nullptr: only query the variant's occupancy limit

@@ -26,6 +26,10 @@
 //                 slowest path once instead of once per bounce, and no ray or
 //                 hit record goes through HBM.
 //   kModePrimary  as kModeDiffuse with the primary shading ((N+1)/2, 0.05).
+//   kModeDiffuse2 as kModeDiffuse with next-event shadow rays: at each hit the
+//                 lane traces an any-hit shadow ray, then the path's next ray.
+//   kModeOccluded work item = a ray of the input array; any-hit query, writes
+//                 the distance of a blocking triangle or -1.
 //
 // Result of a query: the hit with the smallest (t, primId) among triangles
 // whose watertight test accepts tnear <= t <= tfar -- the same answer as
@@ -39,7 +43,10 @@ namespace {
 constexpr int kStackLds = 16;      // default LDS entries per lane (power of two)
 constexpr int kRefill = 24;        // default: refill when this many lanes are idle
 
-enum { kModeTrace = 0, kModeDiffuse = 1, kModePrimary = 2, kModeBatch = 3, kModeRgb = 4, kModeAdjoint = 5 };
+enum {
+    kModeTrace = 0, kModeDiffuse = 1, kModePrimary = 2, kModeBatch = 3, kModeRgb = 4, kModeAdjoint = 5,
+    kModeDiffuse2 = 6, kModeOccluded = 7
+};
 
 __device__ __forceinline__ float byte_f(uint32_t w, int k)
 {
@@ -53,12 +60,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                                                   uint2* spill, int stack_entries, int refill, uint32_t* err)
 {
     static_assert((kStack & (kStack - 1)) == 0, "LDS stack depth must be a power of two");
+    constexpr bool kRayQuery = kMode == kModeTrace || kMode == kModeOccluded;
     __shared__ uint32_t st_node[kStack * kBlock];
     __shared__ float st_t[kStack * kBlock];
     // rays per depth: an LDS histogram in the instrumented variant only, so
     // the timed variant keeps 32 KiB of LDS per block (5 blocks per CU)
     uint32_t* depth_hist = nullptr;
-    if constexpr (kCount && kMode != kModeTrace && kMode != kModeBatch) {
+    if constexpr (kCount && !kRayQuery && kMode != kModeBatch) {
         __shared__ uint32_t hist[kMaxQueueDepth];
         depth_hist = hist;
     }
@@ -67,17 +75,18 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const uint64_t lane_lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t count = kMode == kModeTrace ? (tl.count_ptr ? *tl.count_ptr : tl.count)
+    const uint32_t count = kRayQuery ? (tl.count_ptr ? *tl.count_ptr : tl.count)
                          : (kMode == kModeBatch ? bl.total_work
                                                 : (kMode == kModeAdjoint ? pl.entries * (uint32_t)pl.spp : pl.entries));
     uint2* my_spill = spill + ((size_t)blockIdx.x * kBlock + tid) * (size_t)stack_entries;
     const int spill_cap = stack_entries;
-    if constexpr (kCount && kMode != kModeTrace && kMode != kModeBatch) {
+    if constexpr (kCount && !kRayQuery && kMode != kModeBatch) {
         if (tid < kMaxQueueDepth) depth_hist[tid] = 0;
         __syncthreads();
     }
 
-    bool active = false;
+    bool active = false;   // the lane holds a ray or path
+    bool tracing = false;  // ... and is traversing (else its ray waits for shading)
     uint32_t ray_idx = 0;
     f3 o{0, 0, 0}, d{0, 0, 0};
     float tmin = 0.0f, tbest = 0.0f;
@@ -111,6 +120,11 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     bool replay = false;
     int path_pass = 0;
     int path_x = 0, path_y = 0;
+    // diffuse2: the current ray is a shadow ray; the path's next ray waits in
+    // (eo, ed); radiance gathered so far
+    bool shadow = false;
+    f3 eo{0, 0, 0}, ed{0, 0, 0};
+    float lsum = 0.0f;
     unsigned long long n_nodes = 0, n_tris = 0, n_spill = 0;
     bool exhausted = false;  // wave-uniform: the work counter has run past count
 
@@ -133,9 +147,159 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
         bu = bv = 0.0f;
         sp = bottom = 0;
         node = 0;
+        tracing = true;
     };
 
     for (;;) {
+        // ---- shade the lanes whose ray has finished.  The traversal loop
+        // hands them over in groups, so this code runs for many lanes at once
+        // instead of stalling every traversal step of the wave for one lane.
+        if (active && !tracing) do {
+            if (kMode == kModeOccluded) {
+                tl.any_t[ray_idx] = best_prim >= 0 ? tbest : -1.0f;
+                active = false;
+                break;
+            }
+            if (kMode == kModeDiffuse2 && shadow) {
+                // shadow ray done: light from the sky unless blocked (MIS weight 1/2)
+                if constexpr (kCount) atomicAdd(&depth_hist[depth], 1u);
+                ++n_rays;
+                if (best_prim < 0) lsum += lw * 0.5f;
+                shadow = false;
+                start_ray(eo, ed, 0.0f, 3.402823466e38f);
+                break;
+            }
+            float t = -1.0f;
+            f3 p0{}, p1{}, p2{};
+            if (best_prim >= 0) {
+                const float* tp = tris + 9 * (size_t)best_rec;
+                p0 = mk3(tp[0], tp[1], tp[2]);
+                p1 = mk3(tp[3], tp[4], tp[5]);
+                p2 = mk3(tp[6], tp[7], tp[8]);
+                t = refine_t(p0, p1, p2, o, d, tbest);
+            }
+            if (kMode == kModeTrace) {
+                tl.hits[ray_idx] = best_prim >= 0
+                                       ? make_float4(t, __int_as_float(best_prim), bu, bv)
+                                       : make_float4(-1.0f, __int_as_float(-1), 0.0f, 0.0f);
+                active = false;
+                break;
+            }
+            if constexpr (kCount && kMode != kModeBatch) atomicAdd(&depth_hist[depth], 1u);
+            ++n_rays;
+            float4* acc = kMode == kModeBatch ? nullptr : pl.accum + pixel;
+            if (kMode == kModeDiffuse2) {
+                // extension ray done (render/diffuse2_integrator_single.h:52-85)
+                bool done = false;
+                if (best_prim < 0) {
+                    lsum += lw * (depth > 0 ? 0.5f : 1.0f);
+                    done = true;
+                } else if (depth == max_depth) {
+                    done = true;
+                }
+                if (done) {
+                    const float4 a = *acc;
+                    *acc = make_float4(a.x + lsum, a.y + lsum, a.z + lsum, a.w + 1.0f);
+                    active = false;
+                    break;
+                }
+                lw *= albedo;
+                const f3 n = face_normal(p0, p1, p2, d);
+                const float s1 = sampler_next(rng);
+                const float s2 = sampler_next(rng);
+                const float s3 = sampler_next(rng);
+                const float s4 = sampler_next(rng);
+                f3 so, sd;
+                diffuse_bounce(o, d, t, n, s1, s2, so, sd);
+                diffuse_bounce(o, d, t, n, s3, s4, eo, ed);
+                ++depth;
+                shadow = true;
+                start_ray(so, sd, 0.0f, 3.402823466e38f);
+                break;
+            }
+            if (kMode == kModePrimary) {
+                f3 c = mk3(0.05f, 0.05f, 0.05f);
+                if (best_prim >= 0) {
+                    const f3 n = face_normal(p0, p1, p2, d);
+                    c = mk3((n.x + 1.0f) * 0.5f, (n.y + 1.0f) * 0.5f, (n.z + 1.0f) * 0.5f);
+                }
+                const float4 a = *acc;
+                *acc = make_float4(a.x + c.x, a.y + c.y, a.z + c.z, a.w + 1.0f);
+                active = false;
+                break;
+            }
+            if (best_prim < 0 || depth == max_depth) {
+                if (kMode == kModeAdjoint) {
+                    if (!replay && best_prim < 0) {
+                        // the path escaped: replay it from its seed to
+                        // hand each hit its share of the gradient
+                        replay = true;
+                        rng = sampler_seed(path_pass, pixel);
+                        const float u1 = sampler_next(rng);
+                        const float u2 = sampler_next(rng);
+                        (void)sampler_next(rng);
+                        (void)sampler_next(rng);
+                        const f3 dir = camera_dir(pl.cam, ((float)path_x + u1) * pl.inv_w,
+                                                  ((float)path_y + u2) * pl.inv_h);
+                        depth = 0;
+                        start_ray(pl.cam.origin, dir, 0.0f, 3.402823466e38f);
+                        continue;
+                    }
+                    active = false;
+                    break;
+                }
+                const float c = best_prim < 0 ? lw : 0.0f;
+                if (kMode == kModeBatch) {
+                    *sample_out = c;
+                } else if (kMode == kModeRgb) {
+                    const float cg = best_prim < 0 ? lg : 0.0f, cb = best_prim < 0 ? lb : 0.0f;
+                    const float4 a = *acc;
+                    *acc = make_float4(a.x + c, a.y + cg, a.z + cb, a.w + 1.0f);
+                } else {
+                    const float4 a = *acc;
+                    *acc = make_float4(a.x + c, a.y + c, a.z + c, a.w + 1.0f);
+                }
+                active = false;
+                break;
+            }
+            // diffuse bounce: the next ray of this path
+            if (kMode == kModeRgb || kMode == kModeAdjoint) {
+                const float* a = pl.albedo_rgb + 3 * (size_t)best_prim;
+                const float ar = a[0], ag = a[1], ab = a[2];
+                if (kMode == kModeAdjoint) {
+                    if (!replay) {
+                        // primal: products of the nonzero albedos, zero counts
+                        if (ar != 0.0f) pr *= ar; else zeros += 1u;
+                        if (ag != 0.0f) pg *= ag; else zeros += 1u << 8;
+                        if (ab != 0.0f) pb *= ab; else zeros += 1u << 16;
+                    } else {
+                        // dL_c/da_c(t) = product of the path's other albedos
+                        float* g = pl.grad + 3 * (size_t)best_prim;
+                        const uint32_t zr = zeros & 0xffu, zg = (zeros >> 8) & 0xffu, zb = zeros >> 16;
+                        const float dr = zr == 0 ? pr / ar : (zr == 1 && ar == 0.0f ? pr : 0.0f);
+                        const float dg = zg == 0 ? pg / ag : (zg == 1 && ag == 0.0f ? pg : 0.0f);
+                        const float db = zb == 0 ? pb / ab : (zb == 1 && ab == 0.0f ? pb : 0.0f);
+                        if (gw_r * dr != 0.0f) atomicAdd(g + 0, gw_r * dr);
+                        if (gw_g * dg != 0.0f) atomicAdd(g + 1, gw_g * dg);
+                        if (gw_b * db != 0.0f) atomicAdd(g + 2, gw_b * db);
+                    }
+                } else {
+                    lw *= ar;
+                    lg *= ag;
+                    lb *= ab;
+                }
+            } else {
+                lw *= albedo;
+            }
+            const f3 n = face_normal(p0, p1, p2, d);
+            const float s1 = sampler_next(rng);
+            const float s2 = sampler_next(rng);
+            f3 no, nd;
+            diffuse_bounce(o, d, t, n, s1, s2, no, nd);
+            ++depth;
+            start_ray(no, nd, 0.0f, 3.402823466e38f);
+        } while (0);
+
         // ---- refill idle lanes (wave-uniform branch)
         const uint64_t idle = __ballot(!active);
         const int n_idle = __popcll(idle);
@@ -143,19 +307,20 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             const int leader = __ffsll((unsigned long long)idle) - 1;
             uint32_t base = 0;
             if (lane == leader)
-                base = atomicAdd(kMode == kModeTrace ? tl.fetch_ptr
+                base = atomicAdd(kRayQuery ? tl.fetch_ptr
                                                      : (kMode == kModeBatch ? bl.work_counter : pl.path_counter),
                                  (uint32_t)n_idle);
             base = __shfl(base, leader, 64);
             if (!active) {
                 const uint32_t my = base + (uint32_t)__popcll(idle & lane_lt);
                 if (my < count) {
-                    if (kMode == kModeTrace) {
+                    if (kRayQuery) {
                         ray_idx = my;
                         const float4 r0 = tl.rays[2 * (size_t)my];
                         const float4 r1 = tl.rays[2 * (size_t)my + 1];
                         if (r1.w < 0.0f) {
-                            tl.hits[my] = make_float4(-1.0f, __int_as_float(-1), 0.0f, 0.0f);
+                            if (kMode == kModeOccluded) tl.any_t[my] = -1.0f;
+                            else tl.hits[my] = make_float4(-1.0f, __int_as_float(-1), 0.0f, 0.0f);
                         } else {
                             start_ray(mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), r0.w, r1.w);
                             active = true;
@@ -226,6 +391,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                             start_ray(pl.cam.origin, dir, 0.0f, 3.402823466e38f);
                             depth = 0;
                             lw = lg = lb = 1.0f;
+                            lsum = 0.0f;
+                            shadow = false;
                             active = true;
                             if (kMode == kModeAdjoint) {
                                 replay = false;
@@ -235,7 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                                 gw_r = g[0] * pl.inv_spp;
                                 gw_g = g[1] * pl.inv_spp;
                                 gw_b = g[2] * pl.inv_spp;
-                                if (gw_r == 0.0f && gw_g == 0.0f && gw_b == 0.0f) active = false;
+                                if (gw_r == 0.0f && gw_g == 0.0f && gw_b == 0.0f) active = tracing = false;
                             }
                         }
                     }
@@ -243,13 +410,18 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             }
             if ((uint64_t)base + (uint64_t)n_idle >= count) exhausted = true;
         }
-        if (__ballot(active) == 0) {
+        const uint64_t busy = __ballot(active);
+        if (busy == 0) {
             if (exhausted) break;
             continue;
         }
 
-        // ---- traverse until too many lanes of the wave are idle
-        while (active) {
+        // ---- traverse until enough lanes wait for shading or for new work:
+        // refill lanes once no more than 64 - refill trace; after the work
+        // counter ran out, shade once half of the remaining lanes wait
+        const int n_act = __popcll(busy);
+        const int keep = exhausted ? n_act - max(1, min(refill, n_act >> 1)) : 64 - refill;
+        while (tracing) {
             if (node < 0) {
                 // pop the next node that can still hold a closer hit
                 while (sp > 0) {
@@ -269,107 +441,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                     if (nt <= tbest) { node = (int32_t)nid; break; }
                 }
                 if (node < 0) {
-                    // ---- the ray is finished
-                    float t = -1.0f;
-                    f3 p0{}, p1{}, p2{};
-                    if (best_prim >= 0) {
-                        const float* tp = tris + 9 * (size_t)best_rec;
-                        p0 = mk3(tp[0], tp[1], tp[2]);
-                        p1 = mk3(tp[3], tp[4], tp[5]);
-                        p2 = mk3(tp[6], tp[7], tp[8]);
-                        t = refine_t(p0, p1, p2, o, d, tbest);
-                    }
-                    if (kMode == kModeTrace) {
-                        tl.hits[ray_idx] = best_prim >= 0
-                                               ? make_float4(t, __int_as_float(best_prim), bu, bv)
-                                               : make_float4(-1.0f, __int_as_float(-1), 0.0f, 0.0f);
-                        active = false;
-                        break;
-                    }
-                    if constexpr (kCount && kMode != kModeBatch) atomicAdd(&depth_hist[depth], 1u);
-                    ++n_rays;
-                    float4* acc = kMode == kModeBatch ? nullptr : pl.accum + pixel;
-                    if (kMode == kModePrimary) {
-                        f3 c = mk3(0.05f, 0.05f, 0.05f);
-                        if (best_prim >= 0) {
-                            const f3 n = face_normal(p0, p1, p2, d);
-                            c = mk3((n.x + 1.0f) * 0.5f, (n.y + 1.0f) * 0.5f, (n.z + 1.0f) * 0.5f);
-                        }
-                        const float4 a = *acc;
-                        *acc = make_float4(a.x + c.x, a.y + c.y, a.z + c.z, a.w + 1.0f);
-                        active = false;
-                        break;
-                    }
-                    if (best_prim < 0 || depth == max_depth) {
-                        if (kMode == kModeAdjoint) {
-                            if (!replay && best_prim < 0) {
-                                // the path escaped: replay it from its seed to
-                                // hand each hit its share of the gradient
-                                replay = true;
-                                rng = sampler_seed(path_pass, pixel);
-                                const float u1 = sampler_next(rng);
-                                const float u2 = sampler_next(rng);
-                                (void)sampler_next(rng);
-                                (void)sampler_next(rng);
-                                const f3 dir = camera_dir(pl.cam, ((float)path_x + u1) * pl.inv_w,
-                                                          ((float)path_y + u2) * pl.inv_h);
-                                depth = 0;
-                                start_ray(pl.cam.origin, dir, 0.0f, 3.402823466e38f);
-                                continue;
-                            }
-                            active = false;
-                            break;
-                        }
-                        const float c = best_prim < 0 ? lw : 0.0f;
-                        if (kMode == kModeBatch) {
-                            *sample_out = c;
-                        } else if (kMode == kModeRgb) {
-                            const float cg = best_prim < 0 ? lg : 0.0f, cb = best_prim < 0 ? lb : 0.0f;
-                            const float4 a = *acc;
-                            *acc = make_float4(a.x + c, a.y + cg, a.z + cb, a.w + 1.0f);
-                        } else {
-                            const float4 a = *acc;
-                            *acc = make_float4(a.x + c, a.y + c, a.z + c, a.w + 1.0f);
-                        }
-                        active = false;
-                        break;
-                    }
-                    // diffuse bounce: the next ray of this path
-                    if (kMode == kModeRgb || kMode == kModeAdjoint) {
-                        const float* a = pl.albedo_rgb + 3 * (size_t)best_prim;
-                        const float ar = a[0], ag = a[1], ab = a[2];
-                        if (kMode == kModeAdjoint) {
-                            if (!replay) {
-                                // primal: products of the nonzero albedos, zero counts
-                                if (ar != 0.0f) pr *= ar; else zeros += 1u;
-                                if (ag != 0.0f) pg *= ag; else zeros += 1u << 8;
-                                if (ab != 0.0f) pb *= ab; else zeros += 1u << 16;
-                            } else {
-                                // dL_c/da_c(t) = product of the path's other albedos
-                                float* g = pl.grad + 3 * (size_t)best_prim;
-                                const uint32_t zr = zeros & 0xffu, zg = (zeros >> 8) & 0xffu, zb = zeros >> 16;
-                                const float dr = zr == 0 ? pr / ar : (zr == 1 && ar == 0.0f ? pr : 0.0f);
-                                const float dg = zg == 0 ? pg / ag : (zg == 1 && ag == 0.0f ? pg : 0.0f);
-                                const float db = zb == 0 ? pb / ab : (zb == 1 && ab == 0.0f ? pb : 0.0f);
-                                if (gw_r * dr != 0.0f) atomicAdd(g + 0, gw_r * dr);
-                                if (gw_g * dg != 0.0f) atomicAdd(g + 1, gw_g * dg);
-                                if (gw_b * db != 0.0f) atomicAdd(g + 2, gw_b * db);
-                            }
-                        } else {
-                            lw *= ar;
-                            lg *= ag;
-                            lb *= ab;
-                        }
-                    } else {
-                        lw *= albedo;
-                    }
-                    const f3 n = face_normal(p0, p1, p2, d);
-                    const float s1 = sampler_next(rng);
-                    const float s2 = sampler_next(rng);
-                    f3 no, nd;
-                    diffuse_bounce(o, d, t, n, s1, s2, no, nd);
-                    ++depth;
-                    start_ray(no, nd, 0.0f, 3.402823466e38f);
+                    tracing = false;  // the ray is finished: shade it in the outer loop
+                    break;
                 }
             }
 
@@ -427,32 +500,43 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             uint32_t hit_int = hits & imask;
             uint32_t hit_leaf = hits & ~imask;
 
-            // ---- leaf slots: test their triangles
-            const uint32_t tri_base = w1.y;
+            // ---- leaf slots: one mask over the node's triangle records
+            // (a node owns at most 8 x kLeafMax <= 32 consecutive records)
+            uint32_t trimask = 0;
             while (hit_leaf) {
                 const int k = __builtin_ctz(hit_leaf);
                 hit_leaf &= hit_leaf - 1;
                 const uint32_t mb = ((k < 4 ? meta0 : meta1) >> (8 * (k & 3))) & 0xffu;
-                const uint32_t first = tri_base + (mb & 31u);
-                const uint32_t cnt = mb >> 5;
-                for (uint32_t j = 0; j < cnt; ++j) {
-                    const uint32_t rec = first + j;
-                    const float* tp = tris + 9 * (size_t)rec;
-                    const f3 p0 = mk3(tp[0], tp[1], tp[2]);
-                    const f3 p1 = mk3(tp[3], tp[4], tp[5]);
-                    const f3 p2 = mk3(tp[6], tp[7], tp[8]);
-                    if (kCount) ++n_tris;
-                    TriCandidate c;
-                    if (tri_test(wr, o, p0, p1, p2, tmin, tbest, c)) {
-                        if (c.t <= tbest) {
-                            const int32_t prim = prim_ids[rec];
-                            if (c.t < tbest || best_prim < 0 || prim < best_prim) {
-                                tbest = c.t;
-                                best_prim = prim;
-                                best_rec = rec;
-                                bu = c.v_num / c.det;
-                                bv = c.w_num / c.det;
-                            }
+                trimask |= ((1u << (mb >> 5)) - 1u) << (mb & 31u);
+            }
+            const uint32_t tri_base = w1.y;
+            while (trimask) {
+                const uint32_t rec = tri_base + (uint32_t)__builtin_ctz(trimask);
+                trimask &= trimask - 1;
+                const float* tp = tris + 9 * (size_t)rec;
+                const f3 p0 = mk3(tp[0], tp[1], tp[2]);
+                const f3 p1 = mk3(tp[3], tp[4], tp[5]);
+                const f3 p2 = mk3(tp[6], tp[7], tp[8]);
+                if (kCount) ++n_tris;
+                TriCandidate c;
+                if (tri_test(wr, o, p0, p1, p2, tmin, tbest, c)) {
+                    if ((kMode == kModeOccluded || (kMode == kModeDiffuse2 && shadow)) && c.t <= tbest) {
+                        // any hit ends the query: drop the stack and the rest of the node
+                        tbest = c.t;
+                        best_prim = 0;
+                        sp = bottom = 0;
+                        trimask = 0;
+                        hit_int = 0;
+                        break;
+                    }
+                    if (c.t <= tbest) {
+                        const int32_t prim = prim_ids[rec];
+                        if (c.t < tbest || best_prim < 0 || prim < best_prim) {
+                            tbest = c.t;
+                            best_prim = prim;
+                            best_rec = rec;
+                            bu = c.v_num / c.det;
+                            bv = c.w_num / c.det;
                         }
                     }
                 }
@@ -491,10 +575,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             }
             node = next;
 
-            if (!exhausted && __popcll(__ballot(1)) <= 64 - refill) break;
+            if (__popcll(__ballot(1)) <= keep) break;
         }
     }
-
     if (kCount) {
         for (int off = 32; off > 0; off >>= 1) {
             n_nodes += __shfl_down(n_nodes, off, 64);
@@ -502,13 +585,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             n_spill += __shfl_down(n_spill, off, 64);
         }
         if (lane == 0) {
-            unsigned long long* st = kMode == kModeTrace ? tl.stats : (kMode == kModeBatch ? bl.stats : pl.stats);
+            unsigned long long* st = kRayQuery ? tl.stats : (kMode == kModeBatch ? bl.stats : pl.stats);
             atomicAdd(&st[0], n_nodes);
             atomicAdd(&st[1], n_tris);
             atomicAdd(&st[2], n_spill);
         }
     }
-    if constexpr (kMode != kModeTrace) {
+    if constexpr (!kRayQuery) {
         unsigned long long r = n_rays;
         for (int off = 32; off > 0; off >>= 1) r += __shfl_down(r, off, 64);
         if (lane == 0 && r) atomicAdd(&(kMode == kModeBatch ? bl.stats : pl.stats)[3], r);
@@ -606,11 +689,20 @@ hipError_t launch_adjoint(prt_hip_ctx* ctx, const PathLaunch& p, bool instrument
     return launch_mode<kModeAdjoint>(ctx, TraceLaunch{}, p, instrumented);
 }
 
-hipError_t launch_paths(prt_hip_ctx* ctx, const PathLaunch& p, bool primary, bool instrumented)
+hipError_t launch_occluded(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumented)
+{
+    return launch_mode<kModeOccluded>(ctx, t, PathLaunch{}, instrumented);
+}
+
+hipError_t launch_paths(prt_hip_ctx* ctx, const PathLaunch& p, int renderer, bool instrumented)
 {
     TraceLaunch t{};
-    return primary ? launch_mode<kModePrimary>(ctx, t, p, instrumented)
-                   : launch_mode<kModeDiffuse>(ctx, t, p, instrumented);
+    switch (renderer) {
+    case PRT_RENDERER_PRIMARY: return launch_mode<kModePrimary>(ctx, t, p, instrumented);
+    case PRT_RENDERER_DIFFUSE: return launch_mode<kModeDiffuse>(ctx, t, p, instrumented);
+    case PRT_RENDERER_DIFFUSE2: return launch_mode<kModeDiffuse2>(ctx, t, p, instrumented);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 const BatchKernelConfig kBatchVariants[kBatchVariantCount] = {

@@ -109,7 +109,8 @@ typedef struct prt_trace_counters {
 
 enum {
     PRT_RENDERER_PRIMARY = 0,   /* "primaryHip": (N+1)/2 on hit, 0.05 on miss */
-    PRT_RENDERER_DIFFUSE = 1    /* "diffuseHip": diffuse path tracing         */
+    PRT_RENDERER_DIFFUSE = 1,   /* "diffuseHip": diffuse path tracing         */
+    PRT_RENDERER_DIFFUSE2 = 2   /* "diffuse2Hip": with next-event shadow rays  */
 };
 
 /* --- context ------------------------------------------------------------ */
@@ -140,6 +141,17 @@ int prt_hip_intersect_closest_device(prt_hip_ctx* ctx, const prt_ray* d_rays, pr
 /* Same query on the instrumented kernel; adds per-ray node/triangle counts. */
 int prt_hip_intersect_counters(prt_hip_ctx* ctx, const prt_ray* rays, prt_hit* hits, int64_t n,
                                prt_trace_counters* out_counters);
+
+/* --- any hit (IntersectorStreamCuda::occluded, core/intersector_stream_cuda.h:33;
+ * OptixIntersectorStreamCuda::occluded, accel/optix/optix_intersector_stream_cuda.cpp:84-89,
+ * hit format HIT_T = AnyHitCuda{float t}, core/ray_cuda.cuh:36-44) ----------
+ * out_t[i] >= 0 (the distance of some blocking triangle in [tnear, tfar]) if
+ * ray i is occluded, -1 otherwise; a ray with tfar < 0 gives -1.  Which
+ * blocking triangle's distance is reported is unspecified; only the sign is
+ * part of the contract, as with the reference's isHit().  Host arrays. */
+int prt_hip_occluded(prt_hip_ctx* ctx, const prt_ray* rays, float* out_t, int64_t n);
+/* Device arrays, on the context's stream; asynchronous. */
+int prt_hip_occluded_device(prt_hip_ctx* ctx, const prt_ray* d_rays, float* d_out_t, int64_t n);
 
 /* --- camera / frame (DeviceCuda::initCamera / initFrame / clearFrame) ---- */
 /* Pinhole camera of a view: RenderWindow::makeCamera

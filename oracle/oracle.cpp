@@ -477,13 +477,31 @@ Closest32 closest_fp32(const Scene& sc, V3 o, V3 d, float tmin, float tmax)
     double bound = tmax;
     walk(sc, d3(o), d3(d), tmin, &bound, [&](int id) {
         float t, vn, wn, det;
-        if (!tri_test_f(w, o, sc.tris[id], tmin, r.t, t, vn, wn, det)) return;
+        if (!tri_test_f(w, o, sc.tris[id], tmin, r.t, t, vn, wn, det) || t > r.t) return;
         if (t < r.t || r.prim < 0 || (t == r.t && id < r.prim)) {
             r.t = t; r.prim = id; r.u = vn / det; r.v = wn / det;
             bound = std::max((double)t * (1.0 + 1e-6), (double)t);
         }
     });
     return r;
+}
+
+// Any-hit query in kernel fp32 numerics: the distance of a triangle that the
+// watertight test accepts with tmin <= t <= tmax, or -1.  Whether a blocker
+// exists does not depend on the visiting order; which one is reported does.
+float occluded_fp32(const Scene& sc, V3 o, V3 d, float tmin, float tmax)
+{
+    WRay w = wsetup(d);
+    double bound = tmax;
+    float found = -1.0f;
+    walk(sc, d3(o), d3(d), tmin, &bound, [&](int id) {
+        if (found >= 0.0f) return;
+        float t, vn, wn, det;
+        if (!tri_test_f(w, o, sc.tris[id], tmin, tmax, t, vn, wn, det) || t > tmax) return;
+        found = t;
+        bound = -1.0;  // no further box can be entered
+    });
+    return found;
 }
 
 // Distance to the winning triangle's plane in double, rounded to float:
@@ -546,7 +564,7 @@ Closest32 closest_fp32_fast(const Scene& sc, V3 o, V3 d)
             for (int i = nd.first; i < nd.first + nd.count; ++i) {
                 int id = sc.order[i];
                 float t, vn, wn, det;
-                if (!tri_test_f(w, o, sc.tris[id], 0.0f, r.t, t, vn, wn, det)) continue;
+                if (!tri_test_f(w, o, sc.tris[id], 0.0f, r.t, t, vn, wn, det) || t > r.t) continue;
                 if (t < r.t || r.prim < 0 || (t == r.t && id < r.prim)) {
                     r.t = t; r.prim = id; r.u = vn / det; r.v = wn / det;
                 }
@@ -629,6 +647,12 @@ HitAny closest_any(const Scene& sc, V3 o, V3 d, int numerics)
     }
     Closest64 c = closest_fp64(sc, o, d, 0.0f, 3.402823466e38f, false);
     return HitAny{c.prim, (float)c.t};
+}
+
+bool occluded_any(const Scene& sc, V3 o, V3 d, int numerics)
+{
+    if (numerics == 0) return occluded_fp32(sc, o, d, 0.0f, 3.402823466e38f) >= 0.0f;
+    return closest_fp64(sc, o, d, 0.0f, 3.402823466e38f, false).prim >= 0;
 }
 
 template <class F>
@@ -802,7 +826,7 @@ void orc_intersect_fp32_brute(void* scp, const float* rays, int64_t n, float* hi
         c.t = r[7];
         for (int id = 0; id < (int)sc.tris.size() && r[7] >= 0.0f; ++id) {
             float t, vn, wn, det;
-            if (!tri_test_f(w, o, sc.tris[id], r[3], c.t, t, vn, wn, det)) continue;
+            if (!tri_test_f(w, o, sc.tris[id], r[3], c.t, t, vn, wn, det) || t > c.t) continue;
             if (t < c.t || c.prim < 0 || (t == c.t && id < c.prim)) {
                 c.t = t; c.prim = id; c.u = vn / det; c.v = wn / det;
             }
@@ -811,6 +835,30 @@ void orc_intersect_fp32_brute(void* scp, const float* rays, int64_t n, float* hi
         if (pid >= 0) { h[0] = refine_t(sc.tris[pid], o, d, c.t); h[2] = c.u; h[3] = c.v; }
         else { h[0] = -1.0f; h[2] = 0.0f; h[3] = 0.0f; }
         std::memcpy(&h[1], &pid, 4);
+    }
+}
+
+// kernel-numerics fp32 any-hit query over the BVH (brute = 1: every
+// triangle); out_t: distance of a blocking triangle or -1 per ray
+void orc_occluded_fp32(void* scp, const float* rays, int64_t n, float* out_t, int brute)
+{
+    const Scene& sc = *static_cast<Scene*>(scp);
+    for (int64_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * (size_t)i;
+        V3 o = v3(r[0], r[1], r[2]), d = v3(r[4], r[5], r[6]);
+        float found = -1.0f;
+        if (r[7] >= 0.0f) {
+            if (!brute) {
+                found = occluded_fp32(sc, o, d, r[3], r[7]);
+            } else {
+                WRay w = wsetup(d);
+                for (int id = 0; id < (int)sc.tris.size() && found < 0.0f; ++id) {
+                    float t, vn, wn, det;
+                    if (tri_test_f(w, o, sc.tris[id], r[3], r[7], t, vn, wn, det) && t <= r[7]) found = t;
+                }
+            }
+        }
+        out_t[i] = found;
     }
 }
 
@@ -845,7 +893,8 @@ void orc_intersect_fp32(void* scp, const float* rays, int64_t n, float* hits, in
 // PrimaryIntegratorSingle), passes pass0 .. pass0+spp-1, 1 sample per pass.
 // accum: 4 floats per pixel (sum r, g, b, weight); sumsq: per-pixel sum of the
 // squared grey value (for the Monte Carlo variance); rays_out: intersect calls.
-// renderer 0 = primary, 1 = diffuse; numerics 0 = kernel fp32, 1 = reference fp64.
+// renderer 0 = primary, 1 = diffuse, 2 = diffuse2 (next-event shadow rays);
+// numerics 0 = kernel fp32, 1 = reference fp64.
 int orc_render(void* scp, const float* cam, int w, int h, int pass0, int spp, int max_depth, float albedo,
                int renderer, int numerics, int threads, float* accum, float* sumsq, int64_t* rays_out)
 {
@@ -880,6 +929,33 @@ int orc_render(void* scp, const float* cam, int w, int h, int pass0, int spp, in
                         } else {
                             col[0] = col[1] = col[2] = 0.05f;
                         }
+                    } else if (renderer == 2) {
+                        // Diffuse2IntegratorSingle::getColor (render/diffuse2_integrator_single.h:42-87)
+                        float lw = 1.0f, L = 0.0f;
+                        int depth = 0;
+                        for (;;) {
+                            HitAny hit = closest_any(sc, org, dir, numerics);
+                            ++local_rays;
+                            if (hit.prim < 0) {
+                                L += lw * (depth > 0 ? 0.5f : 1.0f);
+                                break;
+                            }
+                            if (depth == max_depth) break;
+                            lw *= albedo;
+                            V3 n = facing_normal(sc.tris[hit.prim], dir);
+                            float s1 = s.next(), s2 = s.next();
+                            V3 so, sd;
+                            bounce(org, dir, hit.t, n, s1, s2, numerics, so, sd);
+                            ++local_rays;
+                            if (!occluded_any(sc, so, sd, numerics)) L += lw * 0.5f;
+                            float s3 = s.next(), s4 = s.next();
+                            V3 no, nd;
+                            bounce(org, dir, hit.t, n, s3, s4, numerics, no, nd);
+                            org = no;
+                            dir = nd;
+                            ++depth;
+                        }
+                        col[0] = col[1] = col[2] = L;
                     } else {
                         float lw = 1.0f;
                         int depth = 0;

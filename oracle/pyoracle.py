@@ -41,6 +41,7 @@ def lib() -> ctypes.CDLL:
         L.orc_ray_edge_distance.argtypes = [P, P, P, i64, P]
         L.orc_intersect_fp32.argtypes = [P, P, i64, P, i]
         L.orc_intersect_fp32_brute.argtypes = [P, P, i64, P]
+        L.orc_occluded_fp32.argtypes = [P, P, i64, P, i]
         L.orc_render.argtypes = [P, P, i, i, i, i, i, f, i, i, i, P, P, P]
         L.orc_render.restype = i
         L.orc_stream_bench.argtypes = [P, P, i, i, i, i, i, i, i, f, i, P, P, P]
@@ -142,6 +143,13 @@ class Scene:
         hits = np.zeros((rays.shape[0], 4), dtype=np.float32)
         lib().orc_intersect_fp32_brute(self.h, _p(rays), rays.shape[0], _p(hits))
         return hits
+
+    def occluded_fp32(self, rays: np.ndarray, brute: bool = False) -> np.ndarray:
+        """Any-hit query: distance of a blocking triangle per ray, or -1."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        out = np.zeros(rays.shape[0], dtype=np.float32)
+        lib().orc_occluded_fp32(self.h, _p(rays), rays.shape[0], _p(out), int(brute))
+        return out
 
     def render(self, cam: np.ndarray, w: int, h: int, spp: int, max_depth: int, albedo: float,
                renderer: int = 1, numerics: int = 0, pass0: int = 0, threads: int = 0) -> dict:
