@@ -98,6 +98,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     float bu = 0.0f, bv = 0.0f;
     int sp = 0, bottom = 0;
     int32_t node = -1;
+    float node_t = 0.0f;        // entry distance of `node`
+    uint32_t pend = 0, pend_base = 0;  // triangle records still to test: pend_base + bit index
     // path state (fused modes)
     int32_t pixel = 0;
     uint32_t rng = 0;
@@ -147,6 +149,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
         bu = bv = 0.0f;
         sp = bottom = 0;
         node = 0;
+        node_t = near;
+        pend = 0;
         tracing = true;
     };
 
@@ -422,114 +426,173 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
         const int n_act = __popcll(busy);
         const int keep = exhausted ? n_act - max(1, min(refill, n_act >> 1)) : 64 - refill;
         while (tracing) {
-            if (node < 0) {
-                // pop the next node that can still hold a closer hit
-                while (sp > 0) {
-                    --sp;
-                    uint32_t nid;
-                    float nt;
-                    if (sp < bottom) {
-                        const uint2 e = my_spill[sp];
-                        bottom = sp;
-                        nid = e.x;
-                        nt = as_f(e.y);
-                    } else {
-                        const int slot = (sp & (kStack - 1)) * kBlock + tid;
-                        nid = st_node[slot];
-                        nt = st_t[slot];
-                    }
-                    if (nt <= tbest) { node = (int32_t)nid; break; }
-                }
+            if (pend == 0) {
+                // ---- node step: the next node that can still hold a closer hit
+                if (node >= 0 && node_t > tbest) node = -1;
                 if (node < 0) {
-                    tracing = false;  // the ray is finished: shade it in the outer loop
-                    break;
+                    while (sp > 0) {
+                        --sp;
+                        uint32_t nid;
+                        float nt;
+                        if (sp < bottom) {
+                            const uint2 e = my_spill[sp];
+                            bottom = sp;
+                            nid = e.x;
+                            nt = as_f(e.y);
+                        } else {
+                            const int slot = (sp & (kStack - 1)) * kBlock + tid;
+                            nid = st_node[slot];
+                            nt = st_t[slot];
+                        }
+                        if (nt <= tbest) { node = (int32_t)nid; break; }
+                    }
+                    if (node < 0) {
+                        tracing = false;  // the ray is finished: shade it in the outer loop
+                        break;
+                    }
+                }
+
+                // ---- visit one BVH8 node
+                const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
+                const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
+                if (kCount) ++n_nodes;
+                const int ex = (int)(int8_t)(w0.w & 0xffu);
+                const int ey = (int)(int8_t)((w0.w >> 8) & 0xffu);
+                const int ez = (int)(int8_t)((w0.w >> 16) & 0xffu);
+                const uint32_t imask = w0.w >> 24;
+                const float ax = __uint_as_float((uint32_t)(ex + 127) << 23) * inv_x;
+                const float ay = __uint_as_float((uint32_t)(ey + 127) << 23) * inv_y;
+                const float az = __uint_as_float((uint32_t)(ez + 127) << 23) * inv_z;
+                const float bx = (as_f(w0.x) - o.x) * inv_x;
+                const float by = (as_f(w0.y) - o.y) * inv_y;
+                const float bz = (as_f(w0.z) - o.z) * inv_z;
+                // rounding slack of fma(q, a, b): 2^-21 (255|a| + |b|)
+                const float slx = 4.76837158203125e-7f * (255.0f * fabsf(ax) + fabsf(bx));
+                const float sly = 4.76837158203125e-7f * (255.0f * fabsf(ay) + fabsf(by));
+                const float slz = 4.76837158203125e-7f * (255.0f * fabsf(az) + fabsf(bz));
+                const float bxn = bx - slx, bxf = bx + slx;
+                const float byn = by - sly, byf = by + sly;
+                const float bzn = bz - slz, bzf = bz + slz;
+                // near/far planes per axis from the direction signs
+                const uint32_t nx0 = neg_x ? w3.z : w2.x, nx1 = neg_x ? w3.w : w2.y;
+                const uint32_t fx0 = neg_x ? w2.x : w3.z, fx1 = neg_x ? w2.y : w3.w;
+                const uint32_t ny0 = neg_y ? w4.x : w2.z, ny1 = neg_y ? w4.y : w2.w;
+                const uint32_t fy0 = neg_y ? w2.z : w4.x, fy1 = neg_y ? w2.w : w4.y;
+                const uint32_t nz0 = neg_z ? w4.z : w3.x, nz1 = neg_z ? w4.w : w3.y;
+                const uint32_t fz0 = neg_z ? w3.x : w4.z, fz1 = neg_z ? w3.y : w4.w;
+                const uint32_t meta0 = w1.z, meta1 = w1.w;
+
+                // branchless child tests; empty slots carry inverted boxes, and a
+                // leaf slot with meta 0 holds no triangle, so no leaf mask is needed
+                uint32_t hits = 0;
+                float tk[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int kb = k & 3;
+                    const uint32_t nxw = k < 4 ? nx0 : nx1, fxw = k < 4 ? fx0 : fx1;
+                    const uint32_t nyw = k < 4 ? ny0 : ny1, fyw = k < 4 ? fy0 : fy1;
+                    const uint32_t nzw = k < 4 ? nz0 : nz1, fzw = k < 4 ? fz0 : fz1;
+                    const float tnx = __builtin_fmaf(byte_f(nxw, kb), ax, bxn);
+                    const float tny = __builtin_fmaf(byte_f(nyw, kb), ay, byn);
+                    const float tnz = __builtin_fmaf(byte_f(nzw, kb), az, bzn);
+                    const float tfx = __builtin_fmaf(byte_f(fxw, kb), ax, bxf);
+                    const float tfy = __builtin_fmaf(byte_f(fyw, kb), ay, byf);
+                    const float tfz = __builtin_fmaf(byte_f(fzw, kb), az, bzf);
+                    const float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
+                    const float tf = fminf(fminf(fminf(tfx, tfy), tfz), tbest);
+                    tk[k] = tn;
+                    hits |= (tn <= tf ? 1u : 0u) << k;
+                }
+                const uint32_t hit_int = hits & imask;
+                uint32_t hit_leaf = hits & ~imask;
+
+                // ---- leaf slots: one pending mask over the node's triangle
+                // records (a node owns at most 8 x kLeafMax <= 32 of them)
+                while (hit_leaf) {
+                    const int k = __builtin_ctz(hit_leaf);
+                    hit_leaf &= hit_leaf - 1;
+                    const uint32_t mb = ((k < 4 ? meta0 : meta1) >> (8 * (k & 3))) & 0xffu;
+                    pend |= ((1u << (mb >> 5)) - 1u) << (mb & 31u);
+                }
+                pend_base = w1.y;
+
+                // ---- internal slots: the nearest is visited next, the others
+                // are pushed with the second nearest on top.  Unrolled over the
+                // 8 slots with per-slot predicates, so a wave pays one pass
+                // whatever the number of hit children per lane.
+                node = -1;
+                if (hit_int) {
+                    // a hit child has a finite entry distance (tn <= tbest), so
+                    // k1 and, with two or more hits, k2 are always hit slots
+                    float t1 = __builtin_inff(), t2 = __builtin_inff();
+                    int k1 = 0, k2 = 0;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const bool h = (hit_int >> k) & 1u;
+                        const float t = tk[k];
+                        const bool lt1 = h && t < t1;
+                        const bool lt2 = h && !lt1 && t < t2;
+                        t2 = lt1 ? t1 : (lt2 ? t : t2);
+                        k2 = lt1 ? k1 : (lt2 ? k : k2);
+                        t1 = lt1 ? t : t1;
+                        k1 = lt1 ? k : k1;
+                    }
+                    const uint32_t child_base = w1.x;
+                    node = (int32_t)(child_base + __popc(imask & ((1u << k1) - 1u)));
+                    node_t = t1;
+                    const uint32_t pushm = hit_int & ~(1u << k1);
+                    if (pushm) {
+                        const int npush = __popc(pushm);
+                        // make room in the LDS ring: move its oldest entries to
+                        // the lane's global slice
+                        bool ok = true;
+                        while (sp + npush - bottom > kStack) {
+                            if (bottom >= spill_cap) { atomicOr(err, 1u); ok = false; break; }
+                            const int bslot = (bottom & (kStack - 1)) * kBlock + tid;
+                            my_spill[bottom] = make_uint2(st_node[bslot], __float_as_uint(st_t[bslot]));
+                            ++bottom;
+                            if (kCount) ++n_spill;
+                        }
+                        if (ok) {
+                            const int klast = 31 - __builtin_clz(pushm);
+                            if (!((pushm >> k2) & 1u)) k2 = klast;
+                            const int pos_k2 = __popc(pushm & ((1u << k2) - 1u));
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) {
+                                if ((pushm >> k) & 1u) {
+                                    int pos = __popc(pushm & ((1u << k) - 1u));
+                                    pos = k == k2 ? npush - 1 : (k == klast ? pos_k2 : pos);
+                                    const int slot = ((sp + pos) & (kStack - 1)) * kBlock + tid;
+                                    st_node[slot] = child_base + __popc(imask & ((1u << k) - 1u));
+                                    st_t[slot] = tk[k];
+                                }
+                            }
+                            sp += npush;
+                        }
+                    }
                 }
             }
 
-            // ---- visit one BVH8 node
-            const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
-            const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
-            if (kCount) ++n_nodes;
-            const int ex = (int)(int8_t)(w0.w & 0xffu);
-            const int ey = (int)(int8_t)((w0.w >> 8) & 0xffu);
-            const int ez = (int)(int8_t)((w0.w >> 16) & 0xffu);
-            const uint32_t imask = w0.w >> 24;
-            const float ax = __uint_as_float((uint32_t)(ex + 127) << 23) * inv_x;
-            const float ay = __uint_as_float((uint32_t)(ey + 127) << 23) * inv_y;
-            const float az = __uint_as_float((uint32_t)(ez + 127) << 23) * inv_z;
-            const float bx = (as_f(w0.x) - o.x) * inv_x;
-            const float by = (as_f(w0.y) - o.y) * inv_y;
-            const float bz = (as_f(w0.z) - o.z) * inv_z;
-            // rounding slack of fma(q, a, b): 2^-21 (255|a| + |b|)
-            const float slx = 4.76837158203125e-7f * (255.0f * fabsf(ax) + fabsf(bx));
-            const float sly = 4.76837158203125e-7f * (255.0f * fabsf(ay) + fabsf(by));
-            const float slz = 4.76837158203125e-7f * (255.0f * fabsf(az) + fabsf(bz));
-            const float bxn = bx - slx, bxf = bx + slx;
-            const float byn = by - sly, byf = by + sly;
-            const float bzn = bz - slz, bzf = bz + slz;
-            // near/far planes per axis from the direction signs
-            const uint32_t nx0 = neg_x ? w3.z : w2.x, nx1 = neg_x ? w3.w : w2.y;
-            const uint32_t fx0 = neg_x ? w2.x : w3.z, fx1 = neg_x ? w2.y : w3.w;
-            const uint32_t ny0 = neg_y ? w4.x : w2.z, ny1 = neg_y ? w4.y : w2.w;
-            const uint32_t fy0 = neg_y ? w2.z : w4.x, fy1 = neg_y ? w2.w : w4.y;
-            const uint32_t nz0 = neg_z ? w4.z : w3.x, nz1 = neg_z ? w4.w : w3.y;
-            const uint32_t fz0 = neg_z ? w3.x : w4.z, fz1 = neg_z ? w3.y : w4.w;
-            const uint32_t meta0 = w1.z, meta1 = w1.w;
-
-            // branchless child tests; empty slots carry inverted boxes, and a
-            // leaf slot with meta 0 holds no triangle, so no leaf mask is needed
-            uint32_t hits = 0;
-            float tk[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int kb = k & 3;
-                const uint32_t nxw = k < 4 ? nx0 : nx1, fxw = k < 4 ? fx0 : fx1;
-                const uint32_t nyw = k < 4 ? ny0 : ny1, fyw = k < 4 ? fy0 : fy1;
-                const uint32_t nzw = k < 4 ? nz0 : nz1, fzw = k < 4 ? fz0 : fz1;
-                const float tnx = __builtin_fmaf(byte_f(nxw, kb), ax, bxn);
-                const float tny = __builtin_fmaf(byte_f(nyw, kb), ay, byn);
-                const float tnz = __builtin_fmaf(byte_f(nzw, kb), az, bzn);
-                const float tfx = __builtin_fmaf(byte_f(fxw, kb), ax, bxf);
-                const float tfy = __builtin_fmaf(byte_f(fyw, kb), ay, byf);
-                const float tfz = __builtin_fmaf(byte_f(fzw, kb), az, bzf);
-                const float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
-                const float tf = fminf(fminf(fminf(tfx, tfy), tfz), tbest);
-                tk[k] = tn;
-                hits |= (tn <= tf ? 1u : 0u) << k;
-            }
-            uint32_t hit_int = hits & imask;
-            uint32_t hit_leaf = hits & ~imask;
-
-            // ---- leaf slots: one mask over the node's triangle records
-            // (a node owns at most 8 x kLeafMax <= 32 consecutive records)
-            uint32_t trimask = 0;
-            while (hit_leaf) {
-                const int k = __builtin_ctz(hit_leaf);
-                hit_leaf &= hit_leaf - 1;
-                const uint32_t mb = ((k < 4 ? meta0 : meta1) >> (8 * (k & 3))) & 0xffu;
-                trimask |= ((1u << (mb >> 5)) - 1u) << (mb & 31u);
-            }
-            const uint32_t tri_base = w1.y;
-            while (trimask) {
-                const uint32_t rec = tri_base + (uint32_t)__builtin_ctz(trimask);
-                trimask &= trimask - 1;
+            if (pend != 0) {
+                // ---- one triangle of the pending group per step, so lanes
+                // with many candidate triangles do not hold up the wave
+                const uint32_t rec = pend_base + (uint32_t)__builtin_ctz(pend);
+                pend &= pend - 1;
                 const float* tp = tris + 9 * (size_t)rec;
                 const f3 p0 = mk3(tp[0], tp[1], tp[2]);
                 const f3 p1 = mk3(tp[3], tp[4], tp[5]);
                 const f3 p2 = mk3(tp[6], tp[7], tp[8]);
                 if (kCount) ++n_tris;
                 TriCandidate c;
-                if (tri_test(wr, o, p0, p1, p2, tmin, tbest, c)) {
-                    if ((kMode == kModeOccluded || (kMode == kModeDiffuse2 && shadow)) && c.t <= tbest) {
-                        // any hit ends the query: drop the stack and the rest of the node
+                if (tri_test(wr, o, p0, p1, p2, tmin, tbest, c) && c.t <= tbest) {
+                    if (kMode == kModeOccluded || (kMode == kModeDiffuse2 && shadow)) {
+                        // any hit ends the query: drop the stack and the pending work
                         tbest = c.t;
                         best_prim = 0;
                         sp = bottom = 0;
-                        trimask = 0;
-                        hit_int = 0;
-                        break;
-                    }
-                    if (c.t <= tbest) {
+                        pend = 0;
+                        node = -1;
+                    } else {
                         const int32_t prim = prim_ids[rec];
                         if (c.t < tbest || best_prim < 0 || prim < best_prim) {
                             tbest = c.t;
@@ -542,42 +605,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                 }
             }
 
-            // ---- internal slots: go to the nearest, push the others
-            int32_t next = -1;
-            float next_t = 0.0f;
-            const uint32_t child_base = w1.x;
-            while (hit_int) {
-                const int k = __builtin_ctz(hit_int);
-                hit_int &= hit_int - 1;
-                const float t = tk[k];
-                if (t > tbest) continue;
-                const int32_t cidx = (int32_t)(child_base + __popc(imask & ((1u << k) - 1u)));
-                int32_t push_n;
-                float push_t;
-                if (next < 0) { next = cidx; next_t = t; continue; }
-                if (t < next_t) {
-                    push_n = next; push_t = next_t;
-                    next = cidx; next_t = t;
-                } else {
-                    push_n = cidx; push_t = t;
-                }
-                if (sp - bottom == kStack) {
-                    if (bottom >= spill_cap) { atomicOr(err, 1u); break; }
-                    const int bslot = (bottom & (kStack - 1)) * kBlock + tid;
-                    my_spill[bottom] = make_uint2(st_node[bslot], __float_as_uint(st_t[bslot]));
-                    ++bottom;
-                    if (kCount) ++n_spill;
-                }
-                const int slot = (sp & (kStack - 1)) * kBlock + tid;
-                st_node[slot] = (uint32_t)push_n;
-                st_t[slot] = push_t;
-                ++sp;
-            }
-            node = next;
-
             if (__popcll(__ballot(1)) <= keep) break;
         }
     }
+
     if (kCount) {
         for (int off = 32; off > 0; off >>= 1) {
             n_nodes += __shfl_down(n_nodes, off, 64);
