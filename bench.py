@@ -219,6 +219,35 @@ def bench_city(pkg, scenes, args) -> dict:
     return out
 
 
+COPY_FLOATS = 1 << 27   # calibration copy: 512 MiB read + 512 MiB written
+
+
+def pmc_pass(pkg, mesh, cam, cfg, args) -> int:
+    """Short run for the rocprofv3 --pmc passes: `--steps` frames of the fused
+    path kernel (the timed kernel), then a plain elementwise copy of
+    COPY_FLOATS floats (torch, `dst = src * 1`), whose FETCH_SIZE / WRITE_SIZE
+    against its known 512 MiB read and written scale the counters to bytes."""
+    import torch
+    w, h, depth, albedo = cfg["width"], cfg["height"], cfg["max_depth"], cfg["albedo"]
+    dev = pkg.HipDevice(0)
+    dev.upload_scene(mesh.idx, mesh.pos)
+    dev.build()
+    dev.set_camera(cam)
+    dev.init_frame(w, h)
+    for _ in range(args.warmup):
+        dev.render("diffuseHip", max_depth=depth, albedo=albedo)
+    rays = sum(dev.render("diffuseHip", max_depth=depth, albedo=albedo)["ray"] for _ in range(args.steps))
+    log(f"pmc pass: {args.steps} fused frames, {rays} rays")
+    dev.close()
+    src = torch.rand(COPY_FLOATS, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    log(f"pmc pass: 3 copies of {COPY_FLOATS * 4} bytes (read) + {COPY_FLOATS * 4} bytes (written)")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,6 +261,8 @@ def main() -> int:
     ap.add_argument("--city-spp", type=int, default=64)
     ap.add_argument("--city-adjoint-spp", type=int, default=8)
     ap.add_argument("--batch-scenes", type=int, default=32)
+    ap.add_argument("--pmc-pass", action="store_true",
+                    help="short run for rocprofv3 --pmc: the fused frames and a calibration copy, no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is "
@@ -246,6 +277,9 @@ def main() -> int:
     mesh = scenes.build_scene(WORKLOAD)
     v = cfg["view"]
     cam = pkg.camera_from_view(v["pos"], v["angle_x"], v["angle_y"], v["fov_y"], w / h)
+
+    if args.pmc_pass:
+        return pmc_pass(pkg, mesh, cam, cfg, args)
 
     dev = pkg.HipDevice(0)
     log(f"device: {dev.info()}; workload {WORKLOAD}: {mesh.tri_count} triangles, {w}x{h}, depth {depth}")
@@ -290,6 +324,14 @@ def main() -> int:
     achieved = kbytes / (kernel_ms * 1e-3) / 1e9
     survey_bytes = wavefront_bytes(rays, cont, paths, n_node, n_tri)
     depth_rays = [sum(s["queue"][k] for s in cstats) // args.steps for k in range(depth + 1)]
+    wsteps = max(1, sum(s["waveSteps"] for s in cstats))
+    lsteps = sum(s["laneSteps"] for s in cstats)
+    occupancy = {"lanes_per_step": round(lsteps / wsteps, 2), "steps_per_ray": round(lsteps / crays, 3),
+                 "shade_passes_per_frame": sum(s["waveShades"] for s in cstats) // args.steps,
+                 "spills_per_frame": sum(s["spills"] for s in cstats) // args.steps}
+    log(f"traversal loop: {occupancy['lanes_per_step']} of 64 lanes per step, "
+        f"{occupancy['steps_per_ray']} steps per ray, {occupancy['shade_passes_per_frame']} shading passes "
+        f"and {occupancy['spills_per_frame']} stack spills per frame")
     log(f"fused path kernel: {n_node:.2f} nodes/ray, {n_tri:.2f} tris/ray, {kernel_ms:.1f} ms over "
         f"{args.steps} launches ({achieved:.0f} GB/s algorithmic); rays per depth {depth_rays}")
 
@@ -326,7 +368,7 @@ def main() -> int:
         "device_mrays": round(rays / (dev_ms * 1e-3) / 1e6, 2),
         "wavefront_mrays": round(wrays / wwall / 1e6, 2),
         "frame_hbm_frac_wavefront_formula": round(survey_bytes / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "nodes_per_ray": round(n_node, 3), "tris_per_ray": round(n_tri, 3),
+        "nodes_per_ray": round(n_node, 3), "tris_per_ray": round(n_tri, 3), "traversal_occupancy": occupancy,
         "rays_per_frame": rays // args.steps, "rays_per_depth": depth_rays,
         "build": {"buildMs": round(build["buildMs"], 3), "buildMprim": round(build["buildMprim"], 2)},
         "batched": batched,

@@ -47,7 +47,9 @@ class PrtBuildStats(ctypes.Structure):
 class PrtRenderStats(ctypes.Structure):
     _fields_ = [("rays", ctypes.c_int64), ("ms", ctypes.c_double), ("mray", ctypes.c_double),
                 ("paths", ctypes.c_int64), ("queue", ctypes.c_int64 * 16), ("trace_ms", ctypes.c_double),
-                ("shade_ms", ctypes.c_double), ("nodes", ctypes.c_int64), ("tris", ctypes.c_int64)]
+                ("shade_ms", ctypes.c_double), ("nodes", ctypes.c_int64), ("tris", ctypes.c_int64),
+                ("spills", ctypes.c_int64), ("wave_steps", ctypes.c_int64), ("lane_steps", ctypes.c_int64),
+                ("wave_shades", ctypes.c_int64)]
 
 
 class PrtTraceCounters(ctypes.Structure):
@@ -309,7 +311,8 @@ class HipDevice:
         self._check(self.lib.prt_hip_render(self.ctx, renderer, max_depth, albedo, ctypes.byref(st)), "render")
         return {"ray": st.rays, "renderMs": st.ms, "mray": st.mray, "paths": st.paths,
                 "queue": [st.queue[k] for k in range(16)], "traceMs": st.trace_ms, "shadeMs": st.shade_ms,
-                "nodes": st.nodes, "tris": st.tris}
+                "nodes": st.nodes, "tris": st.tris, "spills": st.spills, "waveSteps": st.wave_steps,
+                "laneSteps": st.lane_steps, "waveShades": st.wave_shades}
 
     def set_pipeline(self, pipeline: str) -> None:
         """'fused' (one persistent path launch per frame) or 'wavefront' (per-depth launches)."""

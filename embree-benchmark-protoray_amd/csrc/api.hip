@@ -548,6 +548,10 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
         if (counting) {
             out->nodes = (int64_t)st[0];
             out->tris = (int64_t)st[1];
+            out->spills = (int64_t)st[2];
+            out->wave_steps = (int64_t)st[4];
+            out->lane_steps = (int64_t)st[5];
+            out->wave_shades = (int64_t)st[6];
         }
     }
     return 0;
@@ -934,6 +938,10 @@ int read_path_stats(prt_hip_ctx* ctx, prt_render_stats* out, int64_t paths)
         if (ctx->profiling & PRT_PROFILE_COUNTERS) {
             out->nodes = (int64_t)st[0];
             out->tris = (int64_t)st[1];
+            out->spills = (int64_t)st[2];
+            out->wave_steps = (int64_t)st[4];
+            out->lane_steps = (int64_t)st[5];
+            out->wave_shades = (int64_t)st[6];
         }
     }
     return 0;

@@ -128,6 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     f3 eo{0, 0, 0}, ed{0, 0, 0};
     float lsum = 0.0f;
     unsigned long long n_nodes = 0, n_tris = 0, n_spill = 0;
+    unsigned long long n_wsteps = 0, n_lsteps = 0, n_wshade = 0;  // instrumented SIMT occupancy
     bool exhausted = false;  // wave-uniform: the work counter has run past count
 
     auto start_ray = [&](f3 org, f3 dir, float near, float far) {
@@ -158,6 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
         // ---- shade the lanes whose ray has finished.  The traversal loop
         // hands them over in groups, so this code runs for many lanes at once
         // instead of stalling every traversal step of the wave for one lane.
+        if (kCount && __ballot(active && !tracing) != 0 && lane == 0) ++n_wshade;
         if (active && !tracing) do {
             if (kMode == kModeOccluded) {
                 tl.any_t[ray_idx] = best_prim >= 0 ? tbest : -1.0f;
@@ -426,6 +428,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
         const int n_act = __popcll(busy);
         const int keep = exhausted ? n_act - max(1, min(refill, n_act >> 1)) : 64 - refill;
         while (tracing) {
+            if (kCount) {
+                ++n_lsteps;
+                if (lane == __ffsll((unsigned long long)__ballot(1)) - 1) ++n_wsteps;
+            }
             if (pend == 0) {
                 // ---- node step: the next node that can still hold a closer hit
                 if (node >= 0 && node_t > tbest) node = -1;
@@ -614,12 +620,18 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             n_nodes += __shfl_down(n_nodes, off, 64);
             n_tris += __shfl_down(n_tris, off, 64);
             n_spill += __shfl_down(n_spill, off, 64);
+            n_wsteps += __shfl_down(n_wsteps, off, 64);
+            n_lsteps += __shfl_down(n_lsteps, off, 64);
+            n_wshade += __shfl_down(n_wshade, off, 64);
         }
         if (lane == 0) {
             unsigned long long* st = kRayQuery ? tl.stats : (kMode == kModeBatch ? bl.stats : pl.stats);
             atomicAdd(&st[0], n_nodes);
             atomicAdd(&st[1], n_tris);
             atomicAdd(&st[2], n_spill);
+            atomicAdd(&st[4], n_wsteps);
+            atomicAdd(&st[5], n_lsteps);
+            atomicAdd(&st[6], n_wshade);
         }
     }
     if constexpr (!kRayQuery) {

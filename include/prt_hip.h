@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PRT_HIP_ABI_VERSION 1
+#define PRT_HIP_ABI_VERSION 2
 
 typedef struct prt_hip_ctx prt_hip_ctx;
 
@@ -96,6 +96,12 @@ typedef struct prt_render_stats {
     double shade_ms;      /* summed device time of the shading launches     */
     int64_t nodes;        /* BVH8 nodes visited, all rays of the frame      */
     int64_t tris;         /* triangle tests, all rays of the frame          */
+    /* fused pipeline with PRT_PROFILE_COUNTERS: SIMT occupancy of the path
+     * kernel's traversal loop                                               */
+    int64_t spills;       /* stack entries moved from LDS to global memory  */
+    int64_t wave_steps;   /* traversal-loop iterations, counted per wave    */
+    int64_t lane_steps;   /* traversal-loop iterations, counted per lane    */
+    int64_t wave_shades;  /* shading passes, counted per wave               */
 } prt_render_stats;
 
 /* Per-ray traversal counters (the PROFILE_MODE nodes/prims of
