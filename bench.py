@@ -328,10 +328,16 @@ def main() -> int:
     lsteps = sum(s["laneSteps"] for s in cstats)
     occupancy = {"lanes_per_step": round(lsteps / wsteps, 2), "steps_per_ray": round(lsteps / crays, 3),
                  "shade_passes_per_frame": sum(s["waveShades"] for s in cstats) // args.steps,
-                 "spills_per_frame": sum(s["spills"] for s in cstats) // args.steps}
+                 "spills_per_frame": sum(s["spills"] for s in cstats) // args.steps,
+                 "drain_step_share": round(sum(s["drainWaveSteps"] for s in cstats) / wsteps, 3),
+                 "drain_lanes_per_step": round(sum(s["drainLaneSteps"] for s in cstats)
+                                               / max(1, sum(s["drainWaveSteps"] for s in cstats)), 2),
+                 "drain_ms_per_frame": round(sum(s["drainMs"] for s in cstats) / args.steps, 4)}
     log(f"traversal loop: {occupancy['lanes_per_step']} of 64 lanes per step, "
         f"{occupancy['steps_per_ray']} steps per ray, {occupancy['shade_passes_per_frame']} shading passes "
-        f"and {occupancy['spills_per_frame']} stack spills per frame")
+        f"and {occupancy['spills_per_frame']} stack spills per frame; {occupancy['drain_step_share']} of the "
+        f"steps after the work ran out, at {occupancy['drain_lanes_per_step']} lanes per step, "
+        f"drain {occupancy['drain_ms_per_frame']} ms per frame")
     log(f"fused path kernel: {n_node:.2f} nodes/ray, {n_tri:.2f} tris/ray, {kernel_ms:.1f} ms over "
         f"{args.steps} launches ({achieved:.0f} GB/s algorithmic); rays per depth {depth_rays}")
 

@@ -49,7 +49,8 @@ class PrtRenderStats(ctypes.Structure):
                 ("paths", ctypes.c_int64), ("queue", ctypes.c_int64 * 16), ("trace_ms", ctypes.c_double),
                 ("shade_ms", ctypes.c_double), ("nodes", ctypes.c_int64), ("tris", ctypes.c_int64),
                 ("spills", ctypes.c_int64), ("wave_steps", ctypes.c_int64), ("lane_steps", ctypes.c_int64),
-                ("wave_shades", ctypes.c_int64)]
+                ("wave_shades", ctypes.c_int64), ("drain_wave_steps", ctypes.c_int64),
+                ("drain_lane_steps", ctypes.c_int64), ("drain_ms", ctypes.c_double)]
 
 
 class PrtTraceCounters(ctypes.Structure):
@@ -312,7 +313,8 @@ class HipDevice:
         return {"ray": st.rays, "renderMs": st.ms, "mray": st.mray, "paths": st.paths,
                 "queue": [st.queue[k] for k in range(16)], "traceMs": st.trace_ms, "shadeMs": st.shade_ms,
                 "nodes": st.nodes, "tris": st.tris, "spills": st.spills, "waveSteps": st.wave_steps,
-                "laneSteps": st.lane_steps, "waveShades": st.wave_shades}
+                "laneSteps": st.lane_steps, "waveShades": st.wave_shades,
+                "drainWaveSteps": st.drain_wave_steps, "drainLaneSteps": st.drain_lane_steps, "drainMs": st.drain_ms}
 
     def set_pipeline(self, pipeline: str) -> None:
         """'fused' (one persistent path launch per frame) or 'wavefront' (per-depth launches)."""

@@ -113,6 +113,15 @@ int ensure_io(prt_hip_ctx* ctx, int64_t n)
     return 0;
 }
 
+// Rate of the device's constant wall clock (wall_clock64), in Hz.
+double wall_clock_hz(prt_hip_ctx* ctx)
+{
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess || khz <= 0)
+        return 1e8;
+    return (double)khz * 1e3;
+}
+
 int intersect_impl(prt_hip_ctx* ctx, const prt_ray* d_rays, prt_hit* d_hits, int64_t n,
                    unsigned long long* d_stats)
 {
@@ -304,7 +313,7 @@ int prt_hip_intersect_counters(prt_hip_ctx* ctx, const prt_ray* rays, prt_hit* h
     if (!rays || !hits || !out) return set_error(ctx, "intersect_counters: null argument");
     if (n == 0) { std::memset(out, 0, sizeof(*out)); return 0; }
     if (ensure_io(ctx, n)) return 1;
-    PRT_TRY(ctx, hipMemsetAsync(ctx->d_stats, 0, 8 * sizeof(unsigned long long), ctx->stream));
+    PRT_TRY(ctx, hipMemsetAsync(ctx->d_stats, 0, 16 * sizeof(unsigned long long), ctx->stream));
     PRT_TRY(ctx, hipMemcpyAsync(ctx->d_io_rays, rays, sizeof(prt_ray) * n, hipMemcpyHostToDevice, ctx->stream));
     if (intersect_impl(ctx, reinterpret_cast<prt_ray*>(ctx->d_io_rays),
                        reinterpret_cast<prt_hit*>(ctx->d_io_hits), n, ctx->d_stats))
@@ -552,6 +561,9 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
             out->wave_steps = (int64_t)st[4];
             out->lane_steps = (int64_t)st[5];
             out->wave_shades = (int64_t)st[6];
+            out->drain_wave_steps = (int64_t)st[7];
+            out->drain_lane_steps = (int64_t)st[8];
+            out->drain_ms = st[10] > st[9] ? (double)(st[10] - st[9]) * 1e3 / wall_clock_hz(ctx) : 0.0;
         }
     }
     return 0;
@@ -920,7 +932,7 @@ int albedo_ready(prt_hip_ctx* ctx, int spp, int max_depth, const char* what)
 
 int read_path_stats(prt_hip_ctx* ctx, prt_render_stats* out, int64_t paths)
 {
-    unsigned long long st[8];
+    unsigned long long st[16];
     uint32_t err = 0;
     PRT_TRY(ctx, hipMemcpyAsync(st, ctx->d_stats, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
     PRT_TRY(ctx, hipMemcpyAsync(&err, ctx->d_counters + 32, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -942,6 +954,9 @@ int read_path_stats(prt_hip_ctx* ctx, prt_render_stats* out, int64_t paths)
             out->wave_steps = (int64_t)st[4];
             out->lane_steps = (int64_t)st[5];
             out->wave_shades = (int64_t)st[6];
+            out->drain_wave_steps = (int64_t)st[7];
+            out->drain_lane_steps = (int64_t)st[8];
+            out->drain_ms = st[10] > st[9] ? (double)(st[10] - st[9]) * 1e3 / wall_clock_hz(ctx) : 0.0;
         }
     }
     return 0;

@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kBlock) void k_frame_begin(uint32_t* counters, uint
                                                         unsigned long long* stats)
 {
     if (threadIdx.x < 33) counters[threadIdx.x] = threadIdx.x == 0 ? n0 : 0u;
-    if (threadIdx.x < 32) stats[threadIdx.x] = 0ull;
+    if (threadIdx.x < 32) stats[threadIdx.x] = threadIdx.x == 9 ? ~0ull : 0ull;  // [9]: a minimum
 }
 
 __device__ __forceinline__ void tri_vertices(const int32_t* idx, const float* pos, int prim, f3& p0,
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(64) void k_batch_begin(uint32_t* counters, unsigned
         counters[32] = 0u;
         counters[48] = 0u;
     }
-    if (threadIdx.x < 8) stats[threadIdx.x] = 0ull;
+    if (threadIdx.x < 16) stats[threadIdx.x] = 0ull;
 }
 
 // accum[job pixel] += the job's samples in pass order, so the result is the

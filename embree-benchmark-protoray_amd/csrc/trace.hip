@@ -129,6 +129,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     float lsum = 0.0f;
     unsigned long long n_nodes = 0, n_tris = 0, n_spill = 0;
     unsigned long long n_wsteps = 0, n_lsteps = 0, n_wshade = 0;  // instrumented SIMT occupancy
+    unsigned long long n_dwsteps = 0, n_dlsteps = 0;              // ... after the work ran out
+    unsigned long long t_exhausted = 0;                           // wall clock when it ran out
     bool exhausted = false;  // wave-uniform: the work counter has run past count
 
     auto start_ray = [&](f3 org, f3 dir, float near, float far) {
@@ -414,7 +416,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                     }
                 }
             }
-            if ((uint64_t)base + (uint64_t)n_idle >= count) exhausted = true;
+            if ((uint64_t)base + (uint64_t)n_idle >= count) {
+                exhausted = true;
+                if (kCount) t_exhausted = wall_clock64();
+            }
         }
         const uint64_t busy = __ballot(active);
         if (busy == 0) {
@@ -429,8 +434,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
         const int keep = exhausted ? n_act - max(1, min(refill, n_act >> 1)) : 64 - refill;
         while (tracing) {
             if (kCount) {
+                const bool lead = lane == __ffsll((unsigned long long)__ballot(1)) - 1;
                 ++n_lsteps;
-                if (lane == __ffsll((unsigned long long)__ballot(1)) - 1) ++n_wsteps;
+                if (lead) ++n_wsteps;
+                if (exhausted) {
+                    ++n_dlsteps;
+                    if (lead) ++n_dwsteps;
+                }
             }
             if (pend == 0) {
                 // ---- node step: the next node that can still hold a closer hit
@@ -623,6 +633,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             n_wsteps += __shfl_down(n_wsteps, off, 64);
             n_lsteps += __shfl_down(n_lsteps, off, 64);
             n_wshade += __shfl_down(n_wshade, off, 64);
+            n_dwsteps += __shfl_down(n_dwsteps, off, 64);
+            n_dlsteps += __shfl_down(n_dlsteps, off, 64);
         }
         if (lane == 0) {
             unsigned long long* st = kRayQuery ? tl.stats : (kMode == kModeBatch ? bl.stats : pl.stats);
@@ -632,6 +644,11 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             atomicAdd(&st[4], n_wsteps);
             atomicAdd(&st[5], n_lsteps);
             atomicAdd(&st[6], n_wshade);
+            atomicAdd(&st[7], n_dwsteps);
+            atomicAdd(&st[8], n_dlsteps);
+            // drain window: earliest exhaustion, latest exit (wall clock ticks)
+            if (t_exhausted) atomicMin(&st[9], t_exhausted);
+            atomicMax(&st[10], (unsigned long long)wall_clock64());
         }
     }
     if constexpr (!kRayQuery) {

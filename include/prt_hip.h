@@ -102,6 +102,10 @@ typedef struct prt_render_stats {
     int64_t wave_steps;   /* traversal-loop iterations, counted per wave    */
     int64_t lane_steps;   /* traversal-loop iterations, counted per lane    */
     int64_t wave_shades;  /* shading passes, counted per wave               */
+    int64_t drain_wave_steps;  /* wave_steps / lane_steps taken after the   */
+    int64_t drain_lane_steps;  /* frame's work counter ran out              */
+    double drain_ms;      /* from the first wave finding the work counter
+                             run out to the last wave's exit (wall clock)   */
 } prt_render_stats;
 
 /* Per-ray traversal counters (the PROFILE_MODE nodes/prims of
