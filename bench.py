@@ -31,6 +31,7 @@ import argparse
 import importlib
 import json
 import os
+import re
 import sys
 import time
 
@@ -41,6 +42,19 @@ PKG = "embree-benchmark-protoray_amd"
 METRIC = "Mrays/s diffuse PT 1080p 1.3M tris: plain vs batched; adjoint Mrays/s; %HBM"
 HBM_PEAK_GBS = 8000.0
 WORKLOAD = "spheres_1080p"
+
+
+def measured_traffic() -> dict | None:
+    """HBM-side bytes per fused frame and the counters around them, from the
+    committed rocprofv3 --pmc summary of this workload (FETCH_SIZE and
+    WRITE_SIZE scaled by a calibration copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "k_trace_fused_spheres_1080p.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r'\{"traffic_bytes_per_launch": \d+[^}]*\}', text)
+    return json.loads(m.group(0)) if m else None
 
 
 def log(*a):
@@ -357,6 +371,7 @@ def main() -> int:
     if not args.no_batched:
         batched = bench_batched(pkg, scenes, dev, mesh, cam, cfg, args, value, wrays / wwall / 1e6)
 
+    traffic = measured_traffic()
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3),
@@ -367,7 +382,14 @@ def main() -> int:
                    "triangles": mesh.tri_count, "width": w, "height": h, "max_depth": depth, "albedo": albedo,
                    "pipeline": "fused"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic["traffic_bytes_per_launch"] if traffic else None,
+                     "traffic_source": ("profiles/k_trace_fused_spheres_1080p.md: rocprofv3 --pmc FETCH_SIZE x "
+                                        f"{traffic['fetch_scale']} + WRITE_SIZE x {traffic['write_scale']} per "
+                                        "frame launch") if traffic else None,
+                     "l2_hit_rate": traffic["l2_hit_rate"] if traffic else None,
+                     "valu_busy": traffic["valu_busy"] if traffic else None,
+                     "valu_lane_utilization": traffic["valu_lane_utilization"] if traffic else None,
                      "kernel": "k_trace<fused diffuse> (persistent path kernel: BVH8 closest hit + shading)",
                      "bytes_per_ray": round(kbytes / rays, 1),
                      "avg_launch_ms": round(kernel_ms / args.steps, 4)},
