@@ -174,19 +174,23 @@ def _ptr(a: np.ndarray) -> ctypes.c_void_p:
 
 
 def camera_struct(cam: np.ndarray) -> PrtCamera:
-    """PrtCamera from 18 floats: origin, imageO, imageDx, imageDy, basisU, basisV."""
+    """PrtCamera from 20 floats: origin, imageO, imageDx, imageDy, basisU,
+    basisV, lensRadius, focalDistance.  18 floats are a pinhole camera."""
     cam = np.asarray(cam, dtype=np.float32).ravel()
     c = PrtCamera()
     for k, name in enumerate(["origin", "image_o", "image_dx", "image_dy", "basis_u", "basis_v"]):
         if 3 * k + 3 <= cam.size:
             getattr(c, name)[:] = [float(x) for x in cam[3 * k:3 * k + 3]]
-    c.lens_radius = 0.0
-    c.focal_distance = 1.0
+    c.lens_radius = float(cam[18]) if cam.size >= 20 else 0.0
+    c.focal_distance = float(cam[19]) if cam.size >= 20 else 1.0
     return c
 
 
-def camera_from_view(position, angle_x: float, angle_y: float, fov_y: float, aspect: float) -> np.ndarray:
-    """18 floats (origin, imageO, imageDx, imageDy, basisU, basisV) of a view's pinhole camera."""
+def camera_from_view(position, angle_x: float, angle_y: float, fov_y: float, aspect: float,
+                     lens_radius: float = 0.0, focal_distance: float = 1.0) -> np.ndarray:
+    """20 floats (origin, imageO, imageDx, imageDy, basisU, basisV, lensRadius,
+    focalDistance) of a view's camera: pinhole when lens_radius is 0, thin lens
+    otherwise (RenderWindow::makeCamera, main/render/render_window.cpp:307-325)."""
     lib = load_library()
     pos = (ctypes.c_float * 3)(*[float(x) for x in position])
     cam = PrtCamera()
@@ -195,6 +199,7 @@ def camera_from_view(position, angle_x: float, angle_y: float, fov_y: float, asp
     out = []
     for name in ["origin", "image_o", "image_dx", "image_dy", "basis_u", "basis_v"]:
         out.extend(getattr(cam, name))
+    out += [lens_radius, focal_distance]
     return np.array(out, dtype=np.float32)
 
 

@@ -390,7 +390,8 @@ int prt_hip_camera_set(prt_hip_ctx* ctx, const prt_camera* camera)
 {
     if (use_device(ctx)) return 1;
     if (!camera) return set_error(ctx, "camera_set: null camera");
-    if (camera->lens_radius != 0.0f) return set_error(ctx, "camera_set: only the pinhole camera is supported");
+    if (!(camera->lens_radius >= 0.0f) || !(camera->focal_distance > 0.0f))
+        return set_error(ctx, "camera_set: lens radius must be >= 0 and focal distance > 0");
     ctx->cam = *camera;
     ctx->cam_ready = true;
     return 0;
@@ -703,7 +704,8 @@ int prt_hip_batch_set_jobs(prt_hip_ctx* ctx, const prt_batch_job* jobs, int n_jo
             return set_error(ctx, "batch_set_jobs: bad image size");
         if (in.spp < 1 || in.max_depth < 0 || in.max_depth + 1 >= kMaxQueueDepth)
             return set_error(ctx, "batch_set_jobs: spp must be >= 1 and max_depth in [0, 14]");
-        if (in.camera.lens_radius != 0.0f) return set_error(ctx, "batch_set_jobs: only the pinhole camera");
+        if (!(in.camera.lens_radius >= 0.0f) || !(in.camera.focal_distance > 0.0f))
+            return set_error(ctx, "batch_set_jobs: lens radius must be >= 0 and focal distance > 0");
         BatchJobDevice& d = dj[j];
         d.cam = to_camera(in.camera);
         d.scene = in.scene;

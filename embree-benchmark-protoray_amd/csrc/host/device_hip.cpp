@@ -130,10 +130,12 @@ Props DeviceHip::queryPixel(int x, int y)
 
 void DeviceHip::initCamera(const Props& props)
 {
-    // the pinhole camera of camera/pinhole_camera.h:34-48; the thin-lens
-    // camera (lensRadius > 0) is not part of this device
+    // PinholeCamera (camera/pinhole_camera.h:34-48) plus the thin-lens
+    // parameters of ThinLensCamera (camera/thin_lens_camera.h:31-35); a lens
+    // radius of 0 renders as the pinhole, as DeviceCuda::initCamera does
+    // (render/device_cuda.cpp:175-200)
     const float lens = props.get("lensRadius", 0.0f);
-    if (lens != 0.0f) throw std::invalid_argument("invalid camera type: thin lens is not supported by -dev hip");
+    const float focus = props.get("focalDistance", 1.0f);
     const Vec3f pos = props.get<Vec3f>("position");
     const Basis3f b = props.get<Basis3f>("basis");
     const float position[3] = {pos.x, pos.y, pos.z};
@@ -141,6 +143,8 @@ void DeviceHip::initCamera(const Props& props)
     check(prt_hip_camera_from_basis(position, basis, props.get<float>("fov"), props.get<float>("aspectRatio"),
                                     &camera_),
           "initCamera");
+    camera_.lens_radius = lens;
+    camera_.focal_distance = focus;
     check(prt_hip_camera_set(ctx_, &camera_), "initCamera");
 }
 

@@ -125,6 +125,8 @@ PRT_HD void sincos_2pi(float v, float& s_out, float& c_out)
 // ---------------------------------------------------------------------------
 struct Camera {
     f3 origin, image_o, image_dx, image_dy;
+    f3 basis_u, basis_v;
+    float lens_radius, focal_distance;
 };
 
 PRT_HD f3 camera_dir(const Camera& c, float ix, float iy)
@@ -134,6 +136,36 @@ PRT_HD f3 camera_dir(const Camera& c, float ix, float iy)
     d.y = (c.image_o.y + ix * c.image_dx.y) + iy * c.image_dy.y;
     d.z = (c.image_o.z + ix * c.image_dx.z) + iy * c.image_dy.z;
     return normalize3(d);
+}
+
+// Camera ray of image position (ix, iy) and lens sample (l1, l2).  Pinhole
+// (lens_radius 0): the origin and normalize(imageO + ix dx + iy dy)
+// (camera/pinhole_camera.h:50-54).  Thin lens: ThinLensCamera::getRay
+// (camera/thin_lens_camera.h:38-44) with uniformSampleDisk
+// (sampling/shape_sampler.h:115-124): lens = (r cos 2 pi l2, r sin 2 pi l2) R,
+// r = sqrt(l1); begin = origin + (lens.x U + lens.y V); end = origin + f
+// (imageO + ix dx + iy dy); direction normalize(end - begin).
+PRT_HD void camera_ray(const Camera& c, float ix, float iy, float l1, float l2, f3& org, f3& dir)
+{
+    if (c.lens_radius == 0.0f) {
+        org = c.origin;
+        dir = camera_dir(c, ix, iy);
+        return;
+    }
+    const float r = sqrtf(l1);
+    float sn, cs;
+    sincos_2pi(l2, sn, cs);
+    const float lx = (r * cs) * c.lens_radius, ly = (r * sn) * c.lens_radius;
+    f3 b;
+    b.x = c.origin.x + (lx * c.basis_u.x + ly * c.basis_v.x);
+    b.y = c.origin.y + (lx * c.basis_u.y + ly * c.basis_v.y);
+    b.z = c.origin.z + (lx * c.basis_u.z + ly * c.basis_v.z);
+    f3 e;
+    e.x = c.origin.x + c.focal_distance * ((c.image_o.x + ix * c.image_dx.x) + iy * c.image_dy.x);
+    e.y = c.origin.y + c.focal_distance * ((c.image_o.y + ix * c.image_dx.y) + iy * c.image_dy.y);
+    e.z = c.origin.z + c.focal_distance * ((c.image_o.z + ix * c.image_dx.z) + iy * c.image_dy.z);
+    org = b;
+    dir = normalize3(mk3(e.x - b.x, e.y - b.y, e.z - b.z));
 }
 
 // ---------------------------------------------------------------------------

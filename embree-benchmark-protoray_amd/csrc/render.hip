@@ -48,12 +48,13 @@ __global__ __launch_bounds__(kBlock) void k_raygen(Camera cam, int width, int he
     uint32_t s = sampler_seed(pass, pixel_id);
     float u1 = sampler_next(s);
     float u2 = sampler_next(s);
-    (void)sampler_next(s);  // lens 2D, drawn for the pinhole too
-    (void)sampler_next(s);
+    const float l1 = sampler_next(s);  // lens 2D, drawn for the pinhole too
+    const float l2 = sampler_next(s);
     float ix = ((float)x + u1) * inv_w;
     float iy = ((float)y + u2) * inv_h;
-    f3 d = camera_dir(cam, ix, iy);
-    rays[2 * (size_t)gid] = make_float4(cam.origin.x, cam.origin.y, cam.origin.z, 0.0f);
+    f3 org, d;
+    camera_ray(cam, ix, iy, l1, l2, org, d);
+    rays[2 * (size_t)gid] = make_float4(org.x, org.y, org.z, 0.0f);
     rays[2 * (size_t)gid + 1] = make_float4(d.x, d.y, d.z, 3.402823466e38f);
     pix[gid] = pixel_id;
     rng[gid] = s;
@@ -267,6 +268,10 @@ Camera to_camera(const prt_camera& c)
     k.image_o = mk3(c.image_o[0], c.image_o[1], c.image_o[2]);
     k.image_dx = mk3(c.image_dx[0], c.image_dx[1], c.image_dx[2]);
     k.image_dy = mk3(c.image_dy[0], c.image_dy[1], c.image_dy[2]);
+    k.basis_u = mk3(c.basis_u[0], c.basis_u[1], c.basis_u[2]);
+    k.basis_v = mk3(c.basis_v[0], c.basis_v[1], c.basis_v[2]);
+    k.lens_radius = c.lens_radius;
+    k.focal_distance = c.focal_distance;
     return k;
 }
 

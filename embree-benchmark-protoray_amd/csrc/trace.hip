@@ -245,12 +245,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                         rng = sampler_seed(path_pass, pixel);
                         const float u1 = sampler_next(rng);
                         const float u2 = sampler_next(rng);
-                        (void)sampler_next(rng);
-                        (void)sampler_next(rng);
-                        const f3 dir = camera_dir(pl.cam, ((float)path_x + u1) * pl.inv_w,
-                                                  ((float)path_y + u2) * pl.inv_h);
+                        const float l1 = sampler_next(rng);
+                        const float l2 = sampler_next(rng);
+                        f3 org, dir;
+                        camera_ray(pl.cam, ((float)path_x + u1) * pl.inv_w, ((float)path_y + u2) * pl.inv_h,
+                                   l1, l2, org, dir);
                         depth = 0;
-                        start_ray(pl.cam.origin, dir, 0.0f, 3.402823466e38f);
+                        start_ray(org, dir, 0.0f, 3.402823466e38f);
                         continue;
                     }
                     active = false;
@@ -363,10 +364,12 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                             rng = sampler_seed(job.pass0 + (int)sample, pixel);
                             const float u1 = sampler_next(rng);
                             const float u2 = sampler_next(rng);
-                            (void)sampler_next(rng);  // lens 2D
-                            (void)sampler_next(rng);
-                            const f3 dir = camera_dir(job.cam, ((float)x + u1) * job.inv_w, ((float)y + u2) * job.inv_h);
-                            start_ray(job.cam.origin, dir, 0.0f, 3.402823466e38f);
+                            const float l1 = sampler_next(rng);  // lens 2D
+                            const float l2 = sampler_next(rng);
+                            f3 org, dir;
+                            camera_ray(job.cam, ((float)x + u1) * job.inv_w, ((float)y + u2) * job.inv_h, l1, l2,
+                                       org, dir);
+                            start_ray(org, dir, 0.0f, 3.402823466e38f);
                             depth = 0;
                             lw = 1.0f;
                             active = true;
@@ -393,10 +396,12 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                             rng = sampler_seed(path_pass, pixel);
                             const float u1 = sampler_next(rng);
                             const float u2 = sampler_next(rng);
-                            (void)sampler_next(rng);  // lens 2D
-                            (void)sampler_next(rng);
-                            const f3 dir = camera_dir(pl.cam, ((float)x + u1) * pl.inv_w, ((float)y + u2) * pl.inv_h);
-                            start_ray(pl.cam.origin, dir, 0.0f, 3.402823466e38f);
+                            const float l1 = sampler_next(rng);  // lens 2D
+                            const float l2 = sampler_next(rng);
+                            f3 org, dir;
+                            camera_ray(pl.cam, ((float)x + u1) * pl.inv_w, ((float)y + u2) * pl.inv_h, l1, l2, org,
+                                       dir);
+                            start_ray(org, dir, 0.0f, 3.402823466e38f);
                             depth = 0;
                             lw = lg = lb = 1.0f;
                             lsum = 0.0f;

@@ -587,12 +587,16 @@ Closest32 closest_fp32_fast(const Scene& sc, V3 o, V3 d)
 
 // ----------------------------------------------------------------- camera
 struct Cam {
-    V3 origin, image_o, image_dx, image_dy;
+    V3 origin, image_o, image_dx, image_dy, basis_u, basis_v;
+    float lens_radius, focal_distance;
 };
 
+// 20 floats: origin, imageO, imageDx, imageDy, basisU, basisV, lensRadius,
+// focalDistance (the prt_camera layout)
 Cam cam_from(const float* c)
 {
-    return Cam{v3(c[0], c[1], c[2]), v3(c[3], c[4], c[5]), v3(c[6], c[7], c[8]), v3(c[9], c[10], c[11])};
+    return Cam{v3(c[0], c[1], c[2]), v3(c[3], c[4], c[5]), v3(c[6], c[7], c[8]), v3(c[9], c[10], c[11]),
+               v3(c[12], c[13], c[14]), v3(c[15], c[16], c[17]), c[18], c[19]};
 }
 
 V3 cam_dir(const Cam& c, float ix, float iy)
@@ -602,6 +606,36 @@ V3 cam_dir(const Cam& c, float ix, float iy)
     d.y = (c.image_o.y + ix * c.image_dx.y) + iy * c.image_dy.y;
     d.z = (c.image_o.z + ix * c.image_dx.z) + iy * c.image_dy.z;
     return normalized(d);
+}
+
+// Camera ray: PinholeCamera::getRay (camera/pinhole_camera.h:50-54) when the
+// lens radius is 0, else ThinLensCamera::getRay (camera/thin_lens_camera.h:38-44):
+// lens = uniformSampleDisk(l1, l2) * R (sampling/shape_sampler.h:115-124),
+// begin = origin + basis * (lens, 0), end = origin + f * (imageO + ix dx + iy dy).
+// numerics 0: the kernel's sin/cos of 2 pi l2; 1: libm sinf/cosf.
+void cam_ray(const Cam& c, float ix, float iy, float l1, float l2, int numerics, V3& org, V3& dir)
+{
+    if (c.lens_radius == 0.0f) {
+        org = c.origin;
+        dir = cam_dir(c, ix, iy);
+        return;
+    }
+    float r = std::sqrt(l1), sn, cs;
+    if (numerics == 0) {
+        sincos_2pi_kernel(l2, sn, cs);
+    } else {
+        float theta = 2.0f * 3.14159265358979323846f * l2;
+        sn = std::sin(theta);
+        cs = std::cos(theta);
+    }
+    float lx = (r * cs) * c.lens_radius, ly = (r * sn) * c.lens_radius;
+    V3 b = v3(c.origin.x + (lx * c.basis_u.x + ly * c.basis_v.x), c.origin.y + (lx * c.basis_u.y + ly * c.basis_v.y),
+              c.origin.z + (lx * c.basis_u.z + ly * c.basis_v.z));
+    V3 e = v3(c.origin.x + c.focal_distance * ((c.image_o.x + ix * c.image_dx.x) + iy * c.image_dy.x),
+              c.origin.y + c.focal_distance * ((c.image_o.y + ix * c.image_dx.y) + iy * c.image_dy.y),
+              c.origin.z + c.focal_distance * ((c.image_o.z + ix * c.image_dx.z) + iy * c.image_dy.z));
+    org = b;
+    dir = normalized(v3(e.x - b.x, e.y - b.y, e.z - b.z));
 }
 
 // ----------------------------------------------------------------- shading
@@ -725,8 +759,8 @@ void orc_camera_from_view(const float* pos, float angle_x, float angle_y, float 
     V3 io = tog(v3(-hw, hh, -1.0f));
     V3 idx = tog(v3(2.0f * hw, 0.0f, 0.0f));
     V3 idy = tog(v3(0.0f, (float)(-2.0 * hh), 0.0f));
-    float vals[18] = {pos[0], pos[1], pos[2], io.x, io.y, io.z, idx.x, idx.y, idx.z,
-                      idy.x, idy.y, idy.z, U2.x, U2.y, U2.z, V2.x, V2.y, V2.z};
+    float vals[20] = {pos[0], pos[1], pos[2], io.x, io.y, io.z, idx.x, idx.y, idx.z,
+                      idy.x, idy.y, idy.z, U2.x, U2.y, U2.z, V2.x, V2.y, V2.z, 0.0f, 1.0f};
     std::memcpy(out, vals, sizeof(vals));
 }
 
@@ -741,11 +775,11 @@ void orc_raygen(const float* cam, int w, int h, int pass, float* rays, uint32_t*
             int pid = x + y * w;
             Sampler s{seed_of(pass, pid)};
             float u1 = s.next(), u2 = s.next();
-            s.next();
-            s.next();
-            V3 d = cam_dir(c, ((float)x + u1) * inv_w, ((float)y + u2) * inv_h);
+            float l1 = s.next(), l2 = s.next();
+            V3 o, d;
+            cam_ray(c, ((float)x + u1) * inv_w, ((float)y + u2) * inv_h, l1, l2, 0, o, d);
             float* r = rays + 8 * (size_t)pid;
-            r[0] = c.origin.x; r[1] = c.origin.y; r[2] = c.origin.z; r[3] = 0.0f;
+            r[0] = o.x; r[1] = o.y; r[2] = o.z; r[3] = 0.0f;
             r[4] = d.x; r[5] = d.y; r[6] = d.z; r[7] = 3.402823466e38f;
             if (states) states[pid] = s.s;
         }
@@ -913,10 +947,9 @@ int orc_render(void* scp, const float* cam, int w, int h, int pass0, int spp, in
                     int pid = x + y * w;
                     Sampler s{seed_of(pass, pid)};
                     float u1 = s.next(), u2 = s.next();
-                    s.next();
-                    s.next();
-                    V3 dir = cam_dir(c, ((float)x + u1) * inv_w, ((float)y + u2) * inv_h);
-                    V3 org = c.origin;
+                    float l1 = s.next(), l2 = s.next();
+                    V3 org, dir;
+                    cam_ray(c, ((float)x + u1) * inv_w, ((float)y + u2) * inv_h, l1, l2, numerics, org, dir);
                     float col[3];
                     if (renderer == 0) {
                         HitAny hit = closest_any(sc, org, dir, numerics);
@@ -1022,10 +1055,10 @@ int orc_stream_bench(void* scp, const float* cam, int w, int h, int row0, int ro
                     int pid = x + y * w;
                     Sampler s{seed_of(pass, pid)};
                     float u1 = s.next(), u2 = s.next();
-                    s.next();
-                    s.next();
-                    V3 d = cam_dir(c, ((float)x + u1) * inv_w, ((float)y + u2) * inv_h);
-                    in.push_back(RayS{c.origin, d, 0.0f, -1, pid, s.s});
+                    float l1 = s.next(), l2 = s.next();
+                    V3 o, d;
+                    cam_ray(c, ((float)x + u1) * inv_w, ((float)y + u2) * inv_h, l1, l2, 0, o, d);
+                    in.push_back(RayS{o, d, 0.0f, -1, pid, s.s});
                 }
             int64_t local = 0;
             float lw = 1.0f;
@@ -1122,10 +1155,9 @@ void* orc_paths_record(void* scp, const float* cam, int w, int h, int pass0, int
             const int x = pid % w, y = pid / w;
             Sampler s{seed_of(pass0 + sample, pid)};
             float u1 = s.next(), u2 = s.next();
-            s.next();
-            s.next();
-            V3 dir = cam_dir(c, ((float)x + u1) * inv_w, ((float)y + u2) * inv_h);
-            V3 org = c.origin;
+            float l1 = s.next(), l2 = s.next();
+            V3 org, dir;
+            cam_ray(c, ((float)x + u1) * inv_w, ((float)y + u2) * inv_h, l1, l2, 0, org, dir);
             std::vector<int32_t>& hl = path_hits[i];
             int depth = 0;
             HitAny hit;

@@ -82,15 +82,32 @@ def sincos_2pi(v: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     return s, c
 
 
+def camera20(cam, lens_radius: float | None = None, focal_distance: float | None = None) -> np.ndarray:
+    """Camera as the 20 floats the oracle reads: origin, imageO, imageDx,
+    imageDy, basisU, basisV, lensRadius, focalDistance.  An 18-float (or
+    shorter) array is a pinhole camera (lens radius 0, focal distance 1)."""
+    c = np.asarray(cam, dtype=np.float32).ravel()
+    out = np.zeros(20, dtype=np.float32)
+    n = min(c.size, 20)
+    out[:n] = c[:n]
+    if c.size < 20:
+        out[18], out[19] = 0.0, 1.0
+    if lens_radius is not None:
+        out[18] = lens_radius
+    if focal_distance is not None:
+        out[19] = focal_distance
+    return out
+
+
 def camera_from_view(pos, angle_x: float, angle_y: float, fov_y: float, aspect: float) -> np.ndarray:
     p = np.asarray(pos, dtype=np.float32)
-    out = np.zeros(18, dtype=np.float32)
+    out = np.zeros(20, dtype=np.float32)
     lib().orc_camera_from_view(_p(p), angle_x, angle_y, fov_y, aspect, _p(out))
     return out
 
 
 def raygen(cam: np.ndarray, w: int, h: int, pass_index: int) -> tuple[np.ndarray, np.ndarray]:
-    cam = np.ascontiguousarray(cam, dtype=np.float32)
+    cam = camera20(cam)
     rays = np.zeros((w * h, 8), dtype=np.float32)
     st = np.zeros(w * h, dtype=np.uint32)
     lib().orc_raygen(_p(cam), w, h, pass_index, _p(rays), _p(st))
@@ -153,7 +170,7 @@ class Scene:
 
     def render(self, cam: np.ndarray, w: int, h: int, spp: int, max_depth: int, albedo: float,
                renderer: int = 1, numerics: int = 0, pass0: int = 0, threads: int = 0) -> dict:
-        cam = np.ascontiguousarray(cam, dtype=np.float32)
+        cam = camera20(cam)
         accum = np.zeros((h, w, 4), dtype=np.float32)
         sumsq = np.zeros((h, w), dtype=np.float32)
         rays = ctypes.c_int64(0)
@@ -163,7 +180,7 @@ class Scene:
 
     def stream_bench(self, cam: np.ndarray, w: int, h: int, row0: int, row1: int, frames: int,
                      max_depth: int, albedo: float, threads: int = 0, pass0: int = 0) -> dict:
-        cam = np.ascontiguousarray(cam, dtype=np.float32)
+        cam = camera20(cam)
         accum = np.zeros((h, w, 4), dtype=np.float32)
         rays = ctypes.c_int64(0)
         secs = ctypes.c_double(0.0)
@@ -178,7 +195,7 @@ class PathTape:
 
     def __init__(self, scene: "Scene", cam: np.ndarray, w: int, h: int, pass0: int, spp: int, max_depth: int,
                  threads: int = 0):
-        cam = np.ascontiguousarray(cam, dtype=np.float32)
+        cam = camera20(cam)
         self.w, self.h, self.tri_count = w, h, scene.idx.shape[0]
         self.h_ = lib().orc_paths_record(scene.h, _p(cam), w, h, pass0, spp, max_depth, threads)
 
