@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "device_hip.h"
+#include "mesh_build.h"
 #include "mesh_io.h"
 
 using namespace prt;
@@ -272,14 +273,77 @@ int render_main(const std::vector<std::string>& args)
     return 0;
 }
 
+// `protoray build <input.obj|input.mesh> [-o output.mesh] [-t mesh]`
+// (mainBuild, main/build/build.cpp:31-130): OBJ through the loader and the
+// mesh builder, a .mesh loaded as is; the result saved as a .mesh blob.
+int build_main(const std::vector<std::string>& args)
+{
+    std::cout << "ProtoRay Build" << std::endl << std::endl;
+    if (args.empty()) {
+        std::cout << "Usage: protoray build [options]" << std::endl;
+        return 0;
+    }
+    std::string type = "mesh", input, output;
+    for (const Option& o : parse_options(args)) {
+        if (o.name.empty()) {
+            if (!input.empty()) {
+                std::cerr << "Error: Multiple inputs" << std::endl;
+                return 1;
+            }
+            input = o.value;
+        } else if (o.name == "o") {
+            if (!output.empty()) {
+                std::cerr << "Error: Multiple outputs" << std::endl;
+                return 1;
+            }
+            output = o.value;
+        } else if (o.name == "t") {
+            type = o.value;
+        } else {
+            std::cout << "Option: " << o.name << " = " << o.value << std::endl;
+        }
+    }
+    const size_t dot = input.find_last_of('.');
+    const size_t slash = input.find_last_of('/');
+    const bool has_ext = dot != std::string::npos && (slash == std::string::npos || dot > slash);
+    const std::string ext = has_ext ? input.substr(dot + 1) : "";
+    if (output.empty()) output = (has_ext ? input.substr(0, dot) : input) + "." + type;
+    try {
+        TriangleMesh mesh;
+        if (ext == "mesh") {
+            mesh = load_mesh(input);
+        } else if (ext == "obj") {
+            int degenerate = 0;
+            const TriangleSoup soup = load_obj(input, &degenerate);
+            if (degenerate > 0) std::cout << "Warning: Removed " << degenerate << " degenerate triangles" << std::endl;
+            std::cout << "Building mesh: " << soup.triangles.size() << " triangles, " << soup.triangles.size() * 3
+                      << " vertices" << std::endl;
+            mesh = build_mesh(soup);
+            std::cout << "Unique vertices: " << mesh.vertex_count() << std::endl;
+        } else {
+            std::cerr << "Error: Unsupported input format" << std::endl;
+            return 1;
+        }
+        if (type != "mesh") {
+            std::cerr << "Error: Invalid type" << std::endl;
+            return 1;
+        }
+        save_mesh(output, mesh);
+    } catch (const std::exception& e) {
+        std::cerr << "Error: " << e.what() << std::endl;
+        return 1;
+    }
+    return 0;
+}
+
 int main(int argc, char* argv[])
 {
-    if (argc < 2 || std::string(argv[1]) != "render") {
-        std::cout << "Usage: protoray render <scene.mesh> [-dev hip] [-r primaryHip|diffuseHip|diffuse2Hip] [-size W,H] "
-                     "[-spp N] [-maxDepth D] [-albedo R] [-view N] [-benchmark prefix]"
-                  << std::endl;
-        return argc < 2 ? 0 : 1;
-    }
-    std::vector<std::string> args(argv + 2, argv + argc);
-    return render_main(args);
+    const std::string cmd = argc < 2 ? "" : argv[1];
+    std::vector<std::string> args(argv + (argc < 2 ? argc : 2), argv + argc);
+    if (cmd == "render") return render_main(args);
+    if (cmd == "build") return build_main(args);
+    std::cout << "Usage: protoray render <scene.mesh> [-dev hip] [-r primaryHip|diffuseHip|diffuse2Hip] [-size W,H] "
+                 "[-spp N] [-maxDepth D] [-albedo R] [-view N] [-benchmark prefix]" << std::endl
+              << "       protoray build <scene.obj|scene.mesh> [-o out.mesh] [-t mesh]" << std::endl;
+    return argc < 2 ? 0 : 1;
 }
