@@ -245,14 +245,44 @@ PRT_HD WatertightRay watertight_setup(f3 d)
     return w;
 }
 
-// Candidate intersection.  Returns false for a miss or for a hit that lies
-// outside [tmin, tbest*(1+2^-20)]; otherwise t = T/det, and (U,V,W,det) are
-// kept so that the caller computes u = V/det, v = W/det only for accepted hits.
+// Distance to a triangle's plane in double, rounded to float:
+// t = ((p0 - o) . n) / (d . n) with n = (p1 - p0) x (p2 - p0).  Differences of
+// floats and their products are exact in double; each sum rounds once, so t
+// is the plane distance to within float rounding even for tiny t, where the
+// fp32 T/det of the watertight test loses most of its digits.  False if the
+// plane is parallel to the ray.
+PRT_HD bool plane_t(f3 p0, f3 p1, f3 p2, f3 o, f3 d, float& t_out)
+{
+    double e1x = (double)p1.x - (double)p0.x, e1y = (double)p1.y - (double)p0.y,
+           e1z = (double)p1.z - (double)p0.z;
+    double e2x = (double)p2.x - (double)p0.x, e2y = (double)p2.y - (double)p0.y,
+           e2z = (double)p2.z - (double)p0.z;
+    double nx = e1y * e2z - e1z * e2y;
+    double ny = e1z * e2x - e1x * e2z;
+    double nz = e1x * e2y - e1y * e2x;
+    double wx = (double)p0.x - (double)o.x, wy = (double)p0.y - (double)o.y,
+           wz = (double)p0.z - (double)o.z;
+    double num = (wx * nx + wy * ny) + wz * nz;
+    double den = ((double)d.x * nx + (double)d.y * ny) + (double)d.z * nz;
+    if (den == 0.0) return false;
+    t_out = (float)(num / den);
+    return true;
+}
+
+// Candidate intersection.  The watertight test decides whether the ray's line
+// crosses the triangle (fp32 edge functions, a zero one recomputed in
+// double); the hit distance is the plane distance of plane_t.  Returns false
+// for a miss, a plane parallel to the ray or a distance outside [tmin, tmax];
+// (V, W, det) are kept so that the caller computes u = V/det, v = W/det only
+// for accepted hits.  Ordering candidates by this t rather than by the fp32
+// T/det makes the choice of the nearest triangle follow the true distances,
+// and keeps the traversal's conservative box culling (slack relative to the
+// box distances) consistent with every candidate's t.
 struct TriCandidate {
     float t, v_num, w_num, det;
 };
 
-PRT_HD bool tri_test(const WatertightRay& w, f3 org, f3 p0, f3 p1, f3 p2, float tmin, float tbest,
+PRT_HD bool tri_test(const WatertightRay& w, f3 org, f3 dir, f3 p0, f3 p1, f3 p2, float tmin, float tmax,
                      TriCandidate& c)
 {
     f3 a = sub3(p0, org), b = sub3(p1, org), cc = sub3(p2, org);
@@ -274,43 +304,14 @@ PRT_HD bool tri_test(const WatertightRay& w, f3 org, f3 p0, f3 p1, f3 p2, float 
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
     float det = (U + V) + W;
     if (det == 0.0f) return false;
-    float azs = w.sz * az_, bzs = w.sz * bz_, czs = w.sz * cz_;
-    float T = (U * azs + V * bzs) + W * czs;
-    float tdet = det < 0.0f ? -T : T;
-    float adet = fabsf(det);
-    if (tdet < tmin * adet) return false;
-    if (tdet > (tbest * adet) * 1.00000095367431641f) return false;  // 1 + 2^-20
-    float t = T / det;
-    if (!(t >= tmin)) return false;
+    float t;
+    if (!plane_t(p0, p1, p2, org, dir, t)) return false;
+    if (!(t >= tmin && t <= tmax)) return false;
     c.t = t;
     c.v_num = V;
     c.w_num = W;
     c.det = det;
     return true;
-}
-
-// Distance to the winning triangle's plane, in double:
-// t = ((p0 - o) . n) / (d . n) with n = (p1 - p0) x (p2 - p0), rounded to
-// float.  Reported instead of the watertight test's fp32 T/det, whose error
-// grows at grazing angles and near the ray origin; the choice of triangle is
-// the watertight test's.  Falls back to t32 if the plane is parallel to the ray.
-PRT_HD float refine_t(f3 p0, f3 p1, f3 p2, f3 o, f3 d, float t32)
-{
-    double e1x = (double)p1.x - (double)p0.x, e1y = (double)p1.y - (double)p0.y,
-           e1z = (double)p1.z - (double)p0.z;
-    double e2x = (double)p2.x - (double)p0.x, e2y = (double)p2.y - (double)p0.y,
-           e2z = (double)p2.z - (double)p0.z;
-    double nx = e1y * e2z - e1z * e2y;
-    double ny = e1z * e2x - e1x * e2z;
-    double nz = e1x * e2y - e1y * e2x;
-    double wx = (double)p0.x - (double)o.x, wy = (double)p0.y - (double)o.y,
-           wz = (double)p0.z - (double)o.z;
-    double num = (wx * nx + wy * ny) + wz * nz;
-    double den = ((double)d.x * nx + (double)d.y * ny) + (double)d.z * nz;
-    if (den == 0.0) return t32;
-    double t = num / den;
-    if (!(t >= 0.0)) return t32;
-    return (float)t;
 }
 
 }  // namespace prt

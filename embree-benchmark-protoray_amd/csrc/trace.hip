@@ -32,9 +32,9 @@
 //                 the distance of a blocking triangle or -1.
 //
 // Result of a query: the hit with the smallest (t, primId) among triangles
-// whose watertight test accepts tnear <= t <= tfar -- the same answer as
-// testing every triangle, whatever the visiting order -- with t then refined
-// in double against the winner's plane.
+// that the watertight test says the ray crosses with tnear <= t <= tfar, t
+// being the plane distance computed in double -- the same answer as testing
+// every triangle, whatever the visiting order.
 #include "prt_internal.h"
 
 namespace prt {
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                 p0 = mk3(tp[0], tp[1], tp[2]);
                 p1 = mk3(tp[3], tp[4], tp[5]);
                 p2 = mk3(tp[6], tp[7], tp[8]);
-                t = refine_t(p0, p1, p2, o, d, tbest);
+                t = tbest;  // plane distance of the winner (tri_test)
             }
             if (kMode == kModeTrace) {
                 tl.hits[ray_idx] = best_prim >= 0
@@ -605,7 +605,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                 const f3 p2 = mk3(tp[6], tp[7], tp[8]);
                 if (kCount) ++n_tris;
                 TriCandidate c;
-                if (tri_test(wr, o, p0, p1, p2, tmin, tbest, c) && c.t <= tbest) {
+                if (tri_test(wr, o, d, p0, p1, p2, tmin, tbest, c)) {
                     if (kMode == kModeOccluded || (kMode == kModeDiffuse2 && shadow)) {
                         // any hit ends the query: drop the stack and the pending work
                         tbest = c.t;

@@ -435,7 +435,33 @@ WRay wsetup(V3 d)
     return w;
 }
 
-bool tri_test_f(const WRay& w, V3 org, const Tri& tr, float tmin, float tbest, float& t_out, float& vn,
+// Plane distance in double, rounded to float: t = ((p0 - o) . n) / (d . n),
+// n = (p1 - p0) x (p2 - p0).  Differences of floats and their products are
+// exact in double; each sum rounds once.  False if the plane is parallel to
+// the ray.
+bool plane_t(const Tri& tr, V3 o, V3 d, float& t_out)
+{
+    double e1x = (double)tr.p1.x - (double)tr.p0.x, e1y = (double)tr.p1.y - (double)tr.p0.y,
+           e1z = (double)tr.p1.z - (double)tr.p0.z;
+    double e2x = (double)tr.p2.x - (double)tr.p0.x, e2y = (double)tr.p2.y - (double)tr.p0.y,
+           e2z = (double)tr.p2.z - (double)tr.p0.z;
+    double nx = e1y * e2z - e1z * e2y;
+    double ny = e1z * e2x - e1x * e2z;
+    double nz = e1x * e2y - e1y * e2x;
+    double wx = (double)tr.p0.x - (double)o.x, wy = (double)tr.p0.y - (double)o.y,
+           wz = (double)tr.p0.z - (double)o.z;
+    double num = (wx * nx + wy * ny) + wz * nz;
+    double den = ((double)d.x * nx + (double)d.y * ny) + (double)d.z * nz;
+    if (den == 0.0) return false;
+    t_out = (float)(num / den);
+    return true;
+}
+
+// Kernel numerics: the watertight test (Woop, Benthin, Wald 2013) in fp32,
+// a zero edge function recomputed in double, decides whether the ray's line
+// crosses the triangle; the hit distance is the double plane distance, and
+// the hit counts if tmin <= t <= tmax.
+bool tri_test_f(const WRay& w, V3 org, V3 dir, const Tri& tr, float tmin, float tmax, float& t_out, float& vn,
                 float& wn, float& det_out)
 {
     V3 a = tr.p0 - org, b = tr.p1 - org, c = tr.p2 - org;
@@ -452,14 +478,9 @@ bool tri_test_f(const WRay& w, V3 org, const Tri& tr, float tmin, float tbest, f
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
     float det = (U + V) + W;
     if (det == 0.0f) return false;
-    float azs = w.sz * az_, bzs = w.sz * bz_, czs = w.sz * cz_;
-    float T = (U * azs + V * bzs) + W * czs;
-    float tdet = det < 0.0f ? -T : T;
-    float adet = std::fabs(det);
-    if (tdet < tmin * adet) return false;
-    if (tdet > (tbest * adet) * 1.00000095367431641f) return false;
-    float t = T / det;
-    if (!(t >= tmin)) return false;
+    float t;
+    if (!plane_t(tr, org, dir, t)) return false;
+    if (!(t >= tmin && t <= tmax)) return false;
     t_out = t; vn = V; wn = W; det_out = det;
     return true;
 }
@@ -477,7 +498,7 @@ Closest32 closest_fp32(const Scene& sc, V3 o, V3 d, float tmin, float tmax)
     double bound = tmax;
     walk(sc, d3(o), d3(d), tmin, &bound, [&](int id) {
         float t, vn, wn, det;
-        if (!tri_test_f(w, o, sc.tris[id], tmin, r.t, t, vn, wn, det) || t > r.t) return;
+        if (!tri_test_f(w, o, d, sc.tris[id], tmin, r.t, t, vn, wn, det)) return;
         if (t < r.t || r.prim < 0 || (t == r.t && id < r.prim)) {
             r.t = t; r.prim = id; r.u = vn / det; r.v = wn / det;
             bound = std::max((double)t * (1.0 + 1e-6), (double)t);
@@ -497,34 +518,11 @@ float occluded_fp32(const Scene& sc, V3 o, V3 d, float tmin, float tmax)
     walk(sc, d3(o), d3(d), tmin, &bound, [&](int id) {
         if (found >= 0.0f) return;
         float t, vn, wn, det;
-        if (!tri_test_f(w, o, sc.tris[id], tmin, tmax, t, vn, wn, det) || t > tmax) return;
+        if (!tri_test_f(w, o, d, sc.tris[id], tmin, tmax, t, vn, wn, det)) return;
         found = t;
         bound = -1.0;  // no further box can be entered
     });
     return found;
-}
-
-// Distance to the winning triangle's plane in double, rounded to float:
-// t = ((p0 - o) . n) / (d . n), n = (p1 - p0) x (p2 - p0).  Differences of
-// floats and their products are exact in double; each sum rounds once.
-// Falls back to the fp32 t if the plane is parallel to the ray.
-float refine_t(const Tri& tr, V3 o, V3 d, float t32)
-{
-    double e1x = (double)tr.p1.x - (double)tr.p0.x, e1y = (double)tr.p1.y - (double)tr.p0.y,
-           e1z = (double)tr.p1.z - (double)tr.p0.z;
-    double e2x = (double)tr.p2.x - (double)tr.p0.x, e2y = (double)tr.p2.y - (double)tr.p0.y,
-           e2z = (double)tr.p2.z - (double)tr.p0.z;
-    double nx = e1y * e2z - e1z * e2y;
-    double ny = e1z * e2x - e1x * e2z;
-    double nz = e1x * e2y - e1y * e2x;
-    double wx = (double)tr.p0.x - (double)o.x, wy = (double)tr.p0.y - (double)o.y,
-           wz = (double)tr.p0.z - (double)o.z;
-    double num = (wx * nx + wy * ny) + wz * nz;
-    double den = ((double)d.x * nx + (double)d.y * ny) + (double)d.z * nz;
-    if (den == 0.0) return t32;
-    double t = num / den;
-    if (!(t >= 0.0)) return t32;
-    return (float)t;
 }
 
 // Ordered fp32 traversal for the CPU timing baseline: float slab tests with
@@ -564,7 +562,7 @@ Closest32 closest_fp32_fast(const Scene& sc, V3 o, V3 d)
             for (int i = nd.first; i < nd.first + nd.count; ++i) {
                 int id = sc.order[i];
                 float t, vn, wn, det;
-                if (!tri_test_f(w, o, sc.tris[id], 0.0f, r.t, t, vn, wn, det) || t > r.t) continue;
+                if (!tri_test_f(w, o, d, sc.tris[id], 0.0f, r.t, t, vn, wn, det)) continue;
                 if (t < r.t || r.prim < 0 || (t == r.t && id < r.prim)) {
                     r.t = t; r.prim = id; r.u = vn / det; r.v = wn / det;
                 }
@@ -677,7 +675,7 @@ HitAny closest_any(const Scene& sc, V3 o, V3 d, int numerics)
 {
     if (numerics == 0) {
         Closest32 c = closest_fp32(sc, o, d, 0.0f, 3.402823466e38f);
-        return HitAny{c.prim, c.prim >= 0 ? refine_t(sc.tris[c.prim], o, d, c.t) : c.t};
+        return HitAny{c.prim, c.t};
     }
     Closest64 c = closest_fp64(sc, o, d, 0.0f, 3.402823466e38f, false);
     return HitAny{c.prim, (float)c.t};
@@ -860,13 +858,13 @@ void orc_intersect_fp32_brute(void* scp, const float* rays, int64_t n, float* hi
         c.t = r[7];
         for (int id = 0; id < (int)sc.tris.size() && r[7] >= 0.0f; ++id) {
             float t, vn, wn, det;
-            if (!tri_test_f(w, o, sc.tris[id], r[3], c.t, t, vn, wn, det) || t > c.t) continue;
+            if (!tri_test_f(w, o, d, sc.tris[id], r[3], c.t, t, vn, wn, det)) continue;
             if (t < c.t || c.prim < 0 || (t == c.t && id < c.prim)) {
                 c.t = t; c.prim = id; c.u = vn / det; c.v = wn / det;
             }
         }
         int32_t pid = c.prim;
-        if (pid >= 0) { h[0] = refine_t(sc.tris[pid], o, d, c.t); h[2] = c.u; h[3] = c.v; }
+        if (pid >= 0) { h[0] = c.t; h[2] = c.u; h[3] = c.v; }
         else { h[0] = -1.0f; h[2] = 0.0f; h[3] = 0.0f; }
         std::memcpy(&h[1], &pid, 4);
     }
@@ -888,7 +886,7 @@ void orc_occluded_fp32(void* scp, const float* rays, int64_t n, float* out_t, in
                 WRay w = wsetup(d);
                 for (int id = 0; id < (int)sc.tris.size() && found < 0.0f; ++id) {
                     float t, vn, wn, det;
-                    if (tri_test_f(w, o, sc.tris[id], r[3], r[7], t, vn, wn, det) && t <= r[7]) found = t;
+                    if (tri_test_f(w, o, d, sc.tris[id], r[3], r[7], t, vn, wn, det)) found = t;
                 }
             }
         }
@@ -911,7 +909,7 @@ void orc_intersect_fp32(void* scp, const float* rays, int64_t n, float* hits, in
             if (r[7] >= 0.0f) {
                 Closest32 c = closest_fp32(sc, v3(r[0], r[1], r[2]), v3(r[4], r[5], r[6]), r[3], r[7]);
                 if (c.prim >= 0) {
-                    h[0] = refine_t(sc.tris[c.prim], v3(r[0], r[1], r[2]), v3(r[4], r[5], r[6]), c.t);
+                    h[0] = c.t;
                     pid = c.prim;
                     h[2] = c.u;
                     h[3] = c.v;
@@ -1066,7 +1064,7 @@ int orc_stream_bench(void* scp, const float* cam, int w, int h, int row0, int ro
                 for (RayS& r : in) {
                     Closest32 hc = closest_fp32_fast(sc, r.o, r.d);
                     r.prim = hc.prim;
-                    r.t = hc.prim >= 0 ? refine_t(sc.tris[hc.prim], r.o, r.d, hc.t) : hc.t;
+                    r.t = hc.t;
                 }
                 local += (int64_t)in.size();
                 // misses first (rayStreamSort), then hits

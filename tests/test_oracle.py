@@ -305,3 +305,36 @@ def test_cosine_hemisphere_bounce(orc, numerics):
     p = inp[:, 0:3].astype(np.float64) + inp[:, 6:7] * inp[:, 3:6].astype(np.float64)
     eps = np.maximum(inp[:, 6], np.abs(p).max(1)) * (0x1F / 2.0 ** 22)
     assert np.allclose(out[:, 0:3], p + eps[:, None] * nd, atol=3e-6)
+
+
+def test_nearest_of_close_parallel_triangles(orc):
+    """Candidates are ordered by the plane distance computed in double: two
+    parallel triangles 1e-5 apart, far from the origin of coordinates
+    (|x| ~ 1000, where fp32 T/det has an absolute error near 1e-4), resolve to
+    the truly nearer one for every ray, and the reported t is the float64
+    plane distance to within float rounding."""
+    rng = np.random.default_rng(9)
+    base = np.array([1000.0, 1000.0, 1000.0])
+    # triangle k lies in the plane z = 1000 + 1e-3 + k * 1e-5 (k = 0 nearer)
+    tris = []
+    for k in range(2):
+        z = 1000.0 + 1e-3 + k * 1e-5
+        tris.append([[base[0] - 5, base[1] - 5, z], [base[0] + 5, base[1] - 5, z], [base[0], base[1] + 5, z]])
+    pos = np.array(tris, np.float32).reshape(-1, 3)
+    z_planes = pos[[0, 3], 2].astype(np.float64)
+    assert z_planes[0] < z_planes[1]
+    idx = np.array([[3, 4, 5], [0, 1, 2]], np.int32)   # the nearer plane gets the larger primId
+    sc = orc.Scene(idx, pos)
+    n = 2000
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:2] = base[:2] + rng.uniform(-1, 1, size=(n, 2))
+    rays[:, 2] = 1000.0
+    d = np.stack([rng.uniform(-0.3, 0.3, n), rng.uniform(-0.3, 0.3, n), np.ones(n)], 1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 4:7] = d
+    rays[:, 7] = np.float32(3.4e38)
+    for hits in (sc.intersect_fp32(rays), sc.intersect_fp32_brute(rays)):
+        prim = hits[:, 1].view(np.int32)
+        assert (prim == 1).all(), f"{int((prim != 1).sum())} rays picked the farther plane"
+        t64 = (z_planes[0] - rays[:, 2].astype(np.float64)) / rays[:, 6].astype(np.float64)
+        assert np.allclose(hits[:, 0], t64, rtol=2 ** -22, atol=0)
