@@ -24,7 +24,11 @@ HEADER_PATH = os.path.join(REPO_DIR, "include", "prt_hip.h")
 RENDERER_PRIMARY = 0
 RENDERER_DIFFUSE = 1
 RENDERER_DIFFUSE2 = 2
-RENDERER_IDS = {"primaryHip": RENDERER_PRIMARY, "diffuseHip": RENDERER_DIFFUSE, "diffuse2Hip": RENDERER_DIFFUSE2}
+RENDERER_FAST = 0x100   # modifier: SimpleShadingContext, each pass overwrites the accumulator
+RENDERER_IDS = {"primaryHip": RENDERER_PRIMARY, "diffuseHip": RENDERER_DIFFUSE, "diffuse2Hip": RENDERER_DIFFUSE2,
+                "primaryFastHip": RENDERER_PRIMARY | RENDERER_FAST,
+                "diffuseFastHip": RENDERER_DIFFUSE | RENDERER_FAST,
+                "diffuse2FastHip": RENDERER_DIFFUSE2 | RENDERER_FAST}
 
 RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tnear", "<f4"), ("dir", "<f4", 3), ("tfar", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_id", "<i4"), ("u", "<f4"), ("v", "<f4")])
@@ -115,6 +119,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "prt_hip_last_error": ([P], ctypes.c_char_p),
         "prt_hip_device_name": ([P, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
         "prt_hip_scene_upload": ([P, P, i64, P, i64], ctypes.c_int),
+        "prt_hip_scene_attributes": ([P, P, P], ctypes.c_int),
+        "prt_hip_batch_scene_attributes": ([P, ctypes.c_int32, P, P], ctypes.c_int),
         "prt_hip_bvh_build": ([P, ctypes.POINTER(PrtBuildStats)], ctypes.c_int),
         "prt_hip_intersect_closest": ([P, P, P, i64], ctypes.c_int),
         "prt_hip_intersect_closest_device": ([P, P, P, i64], ctypes.c_int),
@@ -171,6 +177,20 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
 
 def _ptr(a: np.ndarray) -> ctypes.c_void_p:
     return ctypes.c_void_p(a.ctypes.data)
+
+
+def _ptr_or_null(a: np.ndarray | None) -> ctypes.c_void_p | None:
+    return None if a is None else _ptr(a)
+
+
+def _attributes(normals, texcoords, vertex_count: int):
+    """Contiguous float32 (V x 3) normals and (V x 2) texture coordinates, or None."""
+    n = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+    t = None if texcoords is None else np.ascontiguousarray(texcoords, dtype=np.float32).reshape(-1, 2)
+    for a in (n, t):
+        if a is not None and a.shape[0] != vertex_count:
+            raise ValueError("one attribute per vertex expected")
+    return n, t
 
 
 def camera_struct(cam: np.ndarray) -> PrtCamera:
@@ -252,6 +272,15 @@ class HipDevice:
         self._check(self.lib.prt_hip_scene_upload(self.ctx, _ptr(idx), idx.shape[0], _ptr(pos), pos.shape[0]),
                     "scene_upload")
         self.tri_count = idx.shape[0]
+        self.vertex_count = pos.shape[0]
+
+    def set_attributes(self, normals: np.ndarray | None = None, texcoords: np.ndarray | None = None) -> None:
+        """Per-vertex normals (V x 3) and texture coordinates (V x 2) of the uploaded
+        scene; the primary/diffuse/diffuse2 renderers then shade with the full
+        ShadingContext.  None removes an attribute."""
+        n, t = _attributes(normals, texcoords, self.vertex_count)
+        self._check(self.lib.prt_hip_scene_attributes(self.ctx, _ptr_or_null(n), _ptr_or_null(t)),
+                    "scene_attributes")
 
     def build(self) -> dict:
         st = PrtBuildStats()
@@ -393,8 +422,18 @@ class HipDevice:
         sid = ctypes.c_int32()
         self._check(self.lib.prt_hip_batch_scene_add(self.ctx, _ptr(idx), idx.shape[0], _ptr(pos), pos.shape[0],
                                                      ctypes.byref(st), ctypes.byref(sid)), "batch_scene_add")
+        if not hasattr(self, "_batch_vertex_counts"):
+            self._batch_vertex_counts = {}
+        self._batch_vertex_counts[sid.value] = pos.shape[0]
         return sid.value, {"buildMs": st.build_ms, "buildMprim": st.build_mprim, "nodes": st.node_count,
                            "depth": st.depth}
+
+    def batch_set_attributes(self, scene: int, normals: np.ndarray | None = None,
+                             texcoords: np.ndarray | None = None) -> None:
+        """Shading attributes of batch scene `scene` (as set_attributes)."""
+        n, t = _attributes(normals, texcoords, self._batch_vertex_counts[scene])
+        self._check(self.lib.prt_hip_batch_scene_attributes(self.ctx, scene, _ptr_or_null(n), _ptr_or_null(t)),
+                    "batch_scene_attributes")
 
     def batch_reset(self) -> None:
         self._check(self.lib.prt_hip_batch_reset(self.ctx), "batch_reset")

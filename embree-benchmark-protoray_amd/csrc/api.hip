@@ -18,9 +18,11 @@ namespace prt {
 
 void SceneBuffers::release()
 {
-    void* ptrs[] = {d_idx, d_pos, d_nodes, d_tris, d_prim_ids};
+    void* ptrs[] = {d_idx, d_pos, d_nodes, d_tris, d_prim_ids, d_normals, d_texcoords};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    d_normals = nullptr;
+    d_texcoords = nullptr;
     d_idx = nullptr;
     d_pos = nullptr;
     d_nodes = nullptr;
@@ -59,6 +61,27 @@ int use_device(prt_hip_ctx* ctx)
 void free_ptr(void* p)
 {
     if (p) (void)hipFree(p);
+}
+
+// Replaces the shading attributes of a scene: 3 floats per vertex of normals
+// and 2 of texture coordinates; a null pointer removes that attribute.
+int set_attributes(prt_hip_ctx* ctx, SceneBuffers& sb, const float* normals3, const float* texcoords2)
+{
+    free_ptr(sb.d_normals);
+    free_ptr(sb.d_texcoords);
+    sb.d_normals = nullptr;
+    sb.d_texcoords = nullptr;
+    const int64_t nv = sb.vertex_count;
+    if (normals3 && nv > 0) {
+        PRT_TRY(ctx, hipMalloc(&sb.d_normals, 12 * nv));
+        PRT_TRY(ctx, hipMemcpyAsync(sb.d_normals, normals3, 12 * nv, hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (texcoords2 && nv > 0) {
+        PRT_TRY(ctx, hipMalloc(&sb.d_texcoords, 8 * nv));
+        PRT_TRY(ctx, hipMemcpyAsync(sb.d_texcoords, texcoords2, 8 * nv, hipMemcpyHostToDevice, ctx->stream));
+    }
+    PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
 }
 
 int ensure_queues(prt_hip_ctx* ctx, int64_t entries)
@@ -238,8 +261,12 @@ int prt_hip_scene_upload(prt_hip_ctx* ctx, const int32_t* idx3, int64_t tri_coun
             return set_error(ctx, "scene_upload: vertex index out of range");
     free_ptr(ctx->scene.d_idx);
     free_ptr(ctx->scene.d_pos);
+    free_ptr(ctx->scene.d_normals);
+    free_ptr(ctx->scene.d_texcoords);
     ctx->scene.d_idx = nullptr;
     ctx->scene.d_pos = nullptr;
+    ctx->scene.d_normals = nullptr;
+    ctx->scene.d_texcoords = nullptr;
     ctx->scene.bvh_ready = false;
     ctx->scene.tri_count = tri_count;
     ctx->scene.vertex_count = vertex_count;
@@ -254,6 +281,13 @@ int prt_hip_scene_upload(prt_hip_ctx* ctx, const int32_t* idx3, int64_t tri_coun
     }
     PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
+}
+
+int prt_hip_scene_attributes(prt_hip_ctx* ctx, const float* normals3, const float* texcoords2)
+{
+    if (use_device(ctx)) return 1;
+    if (!ctx->scene.d_idx) return set_error(ctx, "scene_attributes: no scene uploaded");
+    return set_attributes(ctx, ctx->scene, normals3, texcoords2);
 }
 
 int prt_hip_bvh_build(prt_hip_ctx* ctx, prt_build_stats* out_stats)
@@ -457,6 +491,8 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
     if (!ctx->scene.bvh_ready) return set_error(ctx, "render: no BVH");
     if (!ctx->cam_ready) return set_error(ctx, "render: no camera");
     if (!ctx->d_accum) return set_error(ctx, "render: no frame");
+    const bool fast = (renderer & PRT_RENDERER_FAST) != 0;
+    renderer &= ~PRT_RENDERER_FAST;
     if (renderer != PRT_RENDERER_PRIMARY && renderer != PRT_RENDERER_DIFFUSE && renderer != PRT_RENDERER_DIFFUSE2)
         return set_error(ctx, "render: invalid renderer type");
     if (renderer == PRT_RENDERER_DIFFUSE2 && ctx->pipeline != PRT_PIPELINE_FUSED)
@@ -490,6 +526,7 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
         p.path_counter = ctx->d_counters + 16;
         p.depth_rays = ctx->d_stats + 16;
         p.stats = ctx->d_stats;
+        p.fast = fast;
         if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[0], ctx->stream));
         PRT_TRY(ctx, launch_paths(ctx, p, renderer, counting));
         if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[1], ctx->stream));
@@ -508,9 +545,9 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
             PRT_TRY(ctx, launch_trace(ctx, t, counting));
             if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[4 * depth + 1], ctx->stream));
             if (renderer == PRT_RENDERER_PRIMARY) {
-                PRT_TRY(ctx, launch_shade_primary(ctx));
+                PRT_TRY(ctx, launch_shade_primary(ctx, fast));
             } else {
-                PRT_TRY(ctx, launch_shade_diffuse(ctx, depth, max_depth, throughput, albedo));
+                PRT_TRY(ctx, launch_shade_diffuse(ctx, depth, max_depth, throughput, fast));
                 throughput *= albedo;
             }
             if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[4 * depth + 2], ctx->stream));
@@ -674,6 +711,19 @@ int prt_hip_batch_scene_add(prt_hip_ctx* ctx, const int32_t* idx3, int64_t tri_c
                            hipMemcpyHostToDevice));
     *out_scene = (int32_t)ctx->batch_scenes.size() - 1;
     return ensure_trace_resources(ctx);
+}
+
+int prt_hip_batch_scene_attributes(prt_hip_ctx* ctx, int32_t scene, const float* normals3,
+                                   const float* texcoords2)
+{
+    if (use_device(ctx)) return 1;
+    if (scene < 0 || scene >= (int32_t)ctx->batch_scenes.size())
+        return set_error(ctx, "batch_scene_attributes: no such scene");
+    if (set_attributes(ctx, ctx->batch_scenes[scene], normals3, texcoords2)) return 1;
+    // the table keeps its address (a captured launch graph refers to it)
+    const SceneDevice d = ctx->batch_scenes[scene].device();
+    PRT_TRY(ctx, hipMemcpy(ctx->d_batch_scenes + scene, &d, sizeof(SceneDevice), hipMemcpyHostToDevice));
+    return 0;
 }
 
 int prt_hip_batch_reset(prt_hip_ctx* ctx)

@@ -2,6 +2,7 @@
 // (render/device_cuda.cpp:40-228): the same props, stats keys and exceptions.
 #include "device_hip.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -37,9 +38,37 @@ void DeviceHip::initScene(const std::string& path, const Props&)
     indices_ = mesh.indices;
     positions_ = mesh.positions();
     bounds_ = mesh.bounds;
+    material_ids_ = mesh.material_ids;
+    material_names_ = mesh.material_names;
     check(prt_hip_scene_upload(ctx_, indices_.data(), mesh.triangle_count(), positions_.data(),
                                mesh.vertex_count()),
           "initScene");
+    // normals follow the position, texture coordinates follow both
+    // (geometry/triangle_mesh.cpp:28-56); repacked as DeviceCuda::initScene
+    // does (render/device_cuda.cpp:144-160)
+    normals_.clear();
+    texcoords_.clear();
+    if (mesh.has_normals || mesh.has_texcoords) {
+        const int64_t nv = mesh.vertex_count();
+        const int stride = mesh.vertex_stride;
+        std::vector<float>& normals = normals_;
+        std::vector<float>& texcoords = texcoords_;
+        if (mesh.has_normals) {
+            normals.resize(3 * (size_t)nv);
+            for (int64_t i = 0; i < nv; ++i)
+                for (int k = 0; k < 3; ++k) normals[3 * i + k] = mesh.vertex_attribs[(size_t)i * stride + 3 + k];
+        }
+        if (mesh.has_texcoords) {
+            const int off = mesh.has_normals ? 6 : 3;
+            texcoords.resize(2 * (size_t)nv);
+            for (int64_t i = 0; i < nv; ++i)
+                for (int k = 0; k < 2; ++k)
+                    texcoords[2 * i + k] = mesh.vertex_attribs[(size_t)i * stride + off + k];
+        }
+        check(prt_hip_scene_attributes(ctx_, mesh.has_normals ? normals.data() : nullptr,
+                                       mesh.has_texcoords ? texcoords.data() : nullptr),
+              "initScene");
+    }
 }
 
 void DeviceHip::initRenderer(const Props& props, Props& stats)
@@ -50,9 +79,14 @@ void DeviceHip::initRenderer(const Props& props, Props& stats)
     if (accel != "bvh8" && accel != "lbvh") throw std::invalid_argument("invalid acceleration structure type");
 
     const std::string type = props.get<std::string>("type");
+    // the reference's CUDA names (render/device_cuda.cpp:79-92) and *Hip
     if (type == "primaryHip" || type == "primary") renderer_ = PRT_RENDERER_PRIMARY;
     else if (type == "diffuseHip" || type == "diffuse") renderer_ = PRT_RENDERER_DIFFUSE;
     else if (type == "diffuse2Hip" || type == "diffuse2") renderer_ = PRT_RENDERER_DIFFUSE2;
+    else if (type == "primaryFastHip" || type == "primaryFast") renderer_ = PRT_RENDERER_PRIMARY | PRT_RENDERER_FAST;
+    else if (type == "diffuseFastHip" || type == "diffuseFast") renderer_ = PRT_RENDERER_DIFFUSE | PRT_RENDERER_FAST;
+    else if (type == "diffuse2FastHip" || type == "diffuse2Fast")
+        renderer_ = PRT_RENDERER_DIFFUSE2 | PRT_RENDERER_FAST;
     else throw std::invalid_argument("invalid renderer type");
 
     std::string sampler = props.get("sampler", "default");
@@ -86,30 +120,71 @@ void DeviceHip::render(Props& stats)
     stats.set("ray", (int64_t)rs.rays);
 }
 
+namespace {
+
+struct V3f {
+    float x, y, z;
+};
+V3f sub(V3f a, V3f b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+V3f crs(V3f a, V3f b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+float dt(V3f a, V3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+V3f nrm(V3f a)
+{
+    const float l = std::sqrt(dt(a, a));
+    return l > 0.0f ? V3f{a.x / l, a.y / l, a.z / l} : a;
+}
+V3f ng3(V3f a) { return {-a.x, -a.y, -a.z}; }
+V3f at3(const std::vector<float>& v, int i) { return {v[3 * (size_t)i], v[3 * (size_t)i + 1], v[3 * (size_t)i + 2]}; }
+
+}  // namespace
+
 Props DeviceHip::queryRay(const Ray& r)
 {
-    // RendererSingle::queryRay (render/renderer_single.h:137-163): primId,
-    // dist, hit point and the ray-facing geometric normal
+    // RendererSingle::queryRay (render/renderer_single.h:137-163): closest hit
+    // on the GPU, then the ShadingContext of TriangleMesh::postIntersect
+    // (geometry/triangle_mesh.cpp:59-153) on the host: material, primId,
+    // distance, hit point, ray-facing Ng, shading normal N, uv and eps
     prt_ray ray{{r.org.x, r.org.y, r.org.z}, 0.0f, {r.dir.x, r.dir.y, r.dir.z}, r.far};
     prt_hit hit{};
     check(prt_hip_intersect_closest(ctx_, &ray, &hit, 1), "queryRay");
     Props res;
     if (hit.t < 0.0f) return res;
     const int32_t* t = &indices_[3 * (size_t)hit.prim_id];
-    float p[3][3];
-    for (int v = 0; v < 3; ++v)
-        for (int k = 0; k < 3; ++k) p[v][k] = positions_[3 * (size_t)t[v] + k];
-    float e1[3], e2[3];
-    for (int k = 0; k < 3; ++k) { e1[k] = p[1][k] - p[0][k]; e2[k] = p[2][k] - p[0][k]; }
-    float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-    const float len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-    if (len > 0.0f) for (float& c : n) c /= len;
-    if (n[0] * r.dir.x + n[1] * r.dir.y + n[2] * r.dir.z > 0.0f) for (float& c : n) c = -c;
+    const V3f p0 = at3(positions_, t[0]), p1 = at3(positions_, t[1]), p2 = at3(positions_, t[2]);
+    const V3f dir{r.dir.x, r.dir.y, r.dir.z};
+    const float b1 = hit.u, b2 = hit.v, b0 = 1.0f - b1 - b2;
+    V3f ng = nrm(crs(sub(p1, p0), sub(p2, p0)));
+    V3f n = ng;
+    if (!normals_.empty()) {
+        const V3f n0 = at3(normals_, t[0]), n1 = at3(normals_, t[1]), n2 = at3(normals_, t[2]);
+        n = nrm({b0 * n0.x + b1 * n1.x + b2 * n2.x, b0 * n0.y + b1 * n1.y + b2 * n2.y,
+                 b0 * n0.z + b1 * n1.z + b2 * n2.z});
+        if (dt(n, ng) < 0.0f) n = ng3(n);
+    }
+    if (dt(ng, dir) > 0.0f) {
+        ng = ng3(ng);
+        n = ng3(n);
+    }
+    Vec3f uv{hit.u, hit.v, 0.0f};
+    if (!texcoords_.empty()) {
+        const float* a = &texcoords_[2 * (size_t)t[0]];
+        const float* b = &texcoords_[2 * (size_t)t[1]];
+        const float* c = &texcoords_[2 * (size_t)t[2]];
+        uv = Vec3f{b0 * a[0] + b1 * b[0] + b2 * c[0], b0 * a[1] + b1 * b[1] + b2 * c[1], 0.0f};
+    }
+    const Vec3f p{r.org.x + hit.t * r.dir.x, r.org.y + hit.t * r.dir.y, r.org.z + hit.t * r.dir.z};
+    const float eps = std::max(hit.t, std::max(std::fabs(p.x), std::max(std::fabs(p.y), std::fabs(p.z)))) *
+                      0x1.fp-18f;
+    const int mat = hit.prim_id < (int32_t)material_ids_.size() ? material_ids_[hit.prim_id] : 0;
+    res.set("mat", mat >= 0 && mat < (int)material_names_.size() ? material_names_[mat] : std::string());
+    res.set("matId", mat);
     res.set("primId", (int64_t)hit.prim_id);
     res.set("dist", hit.t);
-    res.set("p", Vec3f{r.org.x + hit.t * r.dir.x, r.org.y + hit.t * r.dir.y, r.org.z + hit.t * r.dir.z});
-    res.set("Ng", Vec3f{n[0], n[1], n[2]});
-    res.set("uv", Vec3f{hit.u, hit.v, 0.0f});
+    res.set("p", p);
+    res.set("Ng", Vec3f{ng.x, ng.y, ng.z});
+    res.set("N", Vec3f{n.x, n.y, n.z});
+    res.set("uv", uv);
+    res.set("eps", eps);
     return res;
 }
 

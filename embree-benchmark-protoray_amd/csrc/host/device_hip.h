@@ -41,6 +41,10 @@ private:
     Box3f bounds_;
     std::vector<int32_t> indices_;
     std::vector<float> positions_;
+    std::vector<float> normals_;     // 3 per vertex, empty without normals
+    std::vector<float> texcoords_;   // 2 per vertex, empty without texcoords
+    std::vector<int32_t> material_ids_;
+    std::vector<std::string> material_names_;
     int renderer_ = PRT_RENDERER_DIFFUSE;
     int max_depth_ = 6;
     float albedo_ = 0.8f;

@@ -44,7 +44,40 @@ struct SceneDevice {
     const float* tris;        // 9 floats per record
     const int32_t* prim_ids;  // original triangle index per record
     int64_t tri_count;
+    // shading attributes (nullptr when the mesh has none): per-vertex normals
+    // (3 floats) and texture coordinates (2 floats), reached through the
+    // uploaded vertex indices (3 per original triangle)
+    const int32_t* idx;
+    const float* normals;
+    const float* texcoords;
 };
+
+// Shading frame of a hit on original triangle `prim` (vertices p0, p1, p2,
+// hit coordinates (bu, bv)) for a ray along dir, from the per-vertex
+// attributes that are present: the full ShadingContext.
+__device__ __forceinline__ Frame attribute_frame(const int32_t* idx, const float* normals,
+                                                 const float* texcoords, int prim, f3 p0, f3 p1, f3 p2,
+                                                 f3 dir, float bu, float bv)
+{
+    CornerAttributes a{};
+    a.has_normals = normals != nullptr;
+    a.has_texcoords = texcoords != nullptr;
+    const int i0 = idx[3 * (size_t)prim], i1 = idx[3 * (size_t)prim + 1], i2 = idx[3 * (size_t)prim + 2];
+    if (a.has_normals) {
+        a.n0 = mk3(normals[3 * (size_t)i0], normals[3 * (size_t)i0 + 1], normals[3 * (size_t)i0 + 2]);
+        a.n1 = mk3(normals[3 * (size_t)i1], normals[3 * (size_t)i1 + 1], normals[3 * (size_t)i1 + 2]);
+        a.n2 = mk3(normals[3 * (size_t)i2], normals[3 * (size_t)i2 + 1], normals[3 * (size_t)i2 + 2]);
+    }
+    if (a.has_texcoords) {
+        a.u0 = texcoords[2 * (size_t)i0];
+        a.v0 = texcoords[2 * (size_t)i0 + 1];
+        a.u1 = texcoords[2 * (size_t)i1];
+        a.v1 = texcoords[2 * (size_t)i1 + 1];
+        a.u2 = texcoords[2 * (size_t)i2];
+        a.v2 = texcoords[2 * (size_t)i2 + 1];
+    }
+    return shading_frame(p0, p1, p2, dir, bu, bv, a);
+}
 
 // One scene on the device: the uploaded mesh and its BVH8.
 struct SceneBuffers {
@@ -54,11 +87,16 @@ struct SceneBuffers {
     Bvh8Node* d_nodes = nullptr;
     float* d_tris = nullptr;
     int32_t* d_prim_ids = nullptr;
+    float* d_normals = nullptr;     // 3 floats per vertex, or nullptr
+    float* d_texcoords = nullptr;   // 2 floats per vertex, or nullptr
     int64_t node_count = 0;
     int32_t bvh_depth = 0;
     bool bvh_ready = false;
     void release();
-    SceneDevice device() const { return SceneDevice{d_nodes, d_tris, d_prim_ids, tri_count}; }
+    SceneDevice device() const
+    {
+        return SceneDevice{d_nodes, d_tris, d_prim_ids, tri_count, d_idx, d_normals, d_texcoords};
+    }
 };
 
 // One job of a batch: a scene, a camera, an image and a sample count.
@@ -199,6 +237,9 @@ struct PathLaunch {
     float inv_spp;
     const float* dldi;               // dLoss/dImage, 3 floats per pixel
     float* grad;                     // dLoss/dAlbedo, 3 floats per triangle (accumulated)
+    // the "Fast" renderers: frame of the geometric normal whatever attributes
+    // the mesh has, and each pass overwrites the accumulator
+    bool fast;
 };
 int ensure_trace_resources(prt_hip_ctx* ctx);
 hipError_t launch_trace(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumented);
@@ -216,9 +257,9 @@ hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t s);
 hipError_t launch_raygen(prt_hip_ctx* ctx, int pass, float4* rays, int32_t* pix, uint32_t* rng);
 hipError_t launch_frame_begin(prt_hip_ctx* ctx, uint32_t entries);
 Camera to_camera(const prt_camera& c);
-hipError_t launch_shade_diffuse(prt_hip_ctx* ctx, int depth, int max_depth, float throughput,
-                                float albedo_unused);
-hipError_t launch_shade_primary(prt_hip_ctx* ctx);
+// fast: the "Fast" renderers (geometric-normal frame, accumulator overwritten)
+hipError_t launch_shade_diffuse(prt_hip_ctx* ctx, int depth, int max_depth, float throughput, bool fast);
+hipError_t launch_shade_primary(prt_hip_ctx* ctx, bool fast);
 hipError_t launch_tonemap(prt_hip_ctx* ctx);
 hipError_t launch_batch_begin(prt_hip_ctx* ctx);
 hipError_t launch_accum_mean_rgb(prt_hip_ctx* ctx, float* image);

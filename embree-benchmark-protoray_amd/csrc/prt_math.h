@@ -169,10 +169,27 @@ PRT_HD void camera_ray(const Camera& c, float ix, float iy, float l1, float l2, 
 }
 
 // ---------------------------------------------------------------------------
-// Diffuse bounce (render/diffuse_integrator_single.h:55-62 with
-// geometry/triangle_mesh.cpp:248-273, core/ray.h:47-52,71-76,
+// Shading frame and diffuse bounce (render/diffuse_integrator_single.h:55-62
+// with geometry/triangle_mesh.cpp:59-153,248-273, core/ray.h:47-52,71-76,
 // math/basis3.h:128-145, sampling/shape_sampler.h:50-74).
 // ---------------------------------------------------------------------------
+struct Frame {
+    f3 u, v, n;
+};
+
+// makeFrame: U from the larger of (0, N.z, -N.y) and (-N.z, 0, N.x), then
+// V = normalize(N x U).
+PRT_HD Frame make_frame(f3 n)
+{
+    f3 u0 = mk3(0.0f, n.z, -n.y);
+    f3 u1 = mk3(-n.z, 0.0f, n.x);
+    Frame f;
+    f.u = normalize3(dot3(u0, u0) > dot3(u1, u1) ? u0 : u1);
+    f.v = normalize3(cross3(n, f.u));
+    f.n = n;
+    return f;
+}
+
 // Geometric normal of the hit triangle, flipped to face against the ray.
 PRT_HD f3 face_normal(f3 p0, f3 p1, f3 p2, f3 dir)
 {
@@ -181,20 +198,73 @@ PRT_HD f3 face_normal(f3 p0, f3 p1, f3 p2, f3 dir)
     return ng;
 }
 
+// Per-vertex shading attributes of the three corners of a hit triangle.
+struct CornerAttributes {
+    bool has_normals, has_texcoords;
+    f3 n0, n1, n2;
+    float u0, v0, u1, v1, u2, v2;  // texture coordinates
+};
+
+// Full shading frame of a hit (TriangleMesh::postIntersect with a
+// ShadingContext, geometry/triangle_mesh.cpp:59-153 and
+// geometry/triangle_mesh_cuda.cuh:41-130).  (bu, bv) are the hit's
+// barycentric coordinates of p1 and p2, so p = (1 - bu - bv) p0 + bu p1 + bv p2.
+//   N: the interpolated vertex normal, turned to the side of the geometric
+//      normal Ng (Ng itself without normals); both are flipped when Ng faces
+//      along the ray.
+//   U, V: with texture coordinates, U = normalize(dp/du) and V = N x U
+//      normalised, then U = V x N (makeFrame when the texture mapping or
+//      N x U is degenerate); without them, makeFrame(N).
+// 1/det is the correctly rounded quotient (the reference's rcp).
+PRT_HD Frame shading_frame(f3 p0, f3 p1, f3 p2, f3 dir, float bu, float bv, const CornerAttributes& a)
+{
+    const float b1 = bu, b2 = bv;
+    const float b0 = 1.0f - b1 - b2;
+    const f3 dp1 = sub3(p1, p0), dp2 = sub3(p2, p0);
+    f3 ng = normalize3(cross3(dp1, dp2));
+    f3 n = ng;
+    if (a.has_normals) {
+        f3 s;
+        s.x = (b0 * a.n0.x + b1 * a.n1.x) + b2 * a.n2.x;
+        s.y = (b0 * a.n0.y + b1 * a.n1.y) + b2 * a.n2.y;
+        s.z = (b0 * a.n0.z + b1 * a.n1.z) + b2 * a.n2.z;
+        n = normalize3(s);
+        if (dot3(n, ng) < 0.0f) n = neg3(n);
+    }
+    if (dot3(ng, dir) > 0.0f) n = neg3(n);
+    if (a.has_texcoords) {
+        const float du1 = a.u1 - a.u0, du2 = a.u2 - a.u0;
+        const float dv1 = a.v1 - a.v0, dv2 = a.v2 - a.v0;
+        const float det = du1 * dv2 - dv1 * du2;
+        if (det != 0.0f) {
+            const float inv_det = 1.0f / det;
+            f3 dpdu;
+            dpdu.x = (dv2 * dp1.x - dv1 * dp2.x) * inv_det;
+            dpdu.y = (dv2 * dp1.y - dv1 * dp2.y) * inv_det;
+            dpdu.z = (dv2 * dp1.z - dv1 * dp2.z) * inv_det;
+            Frame f;
+            f.n = n;
+            f.u = normalize3(dpdu);
+            f.v = cross3(n, f.u);
+            if (dot3(f.v, f.v) > 0.0f) {
+                f.v = normalize3(f.v);
+                f.u = cross3(f.v, n);
+                return f;
+            }
+        }
+    }
+    return make_frame(n);
+}
+
 // New ray of a diffuse bounce at hit distance t along (org, dir): cosine-
-// weighted direction around n, origin offset by eps along it, tnear 0.
-PRT_HD void diffuse_bounce(f3 org, f3 dir, float t, f3 n, float s1, float s2, f3& org_o, f3& dir_o)
+// weighted direction in frame f, origin offset by eps along it, tnear 0.
+PRT_HD void diffuse_bounce(f3 org, f3 dir, float t, const Frame& f, float s1, float s2, f3& org_o, f3& dir_o)
 {
     f3 p;
     p.x = org.x + t * dir.x;
     p.y = org.y + t * dir.y;
     p.z = org.z + t * dir.z;
     float eps = fmaxf(t, max3f(fabsf(p.x), fabsf(p.y), fabsf(p.z))) * 0x1.fp-18f;
-    // makeFrame
-    f3 u0 = mk3(0.0f, n.z, -n.y);
-    f3 u1 = mk3(-n.z, 0.0f, n.x);
-    f3 u = normalize3(dot3(u0, u0) > dot3(u1, u1) ? u0 : u1);
-    f3 v = normalize3(cross3(n, u));
     // cosineSampleHemisphere
     float cos_t = sqrtf(s1);
     float sin_t = sqrtf(1.0f - s1);
@@ -204,13 +274,20 @@ PRT_HD void diffuse_bounce(f3 org, f3 dir, float t, f3 n, float s1, float s2, f3
     float ly = sp * sin_t;
     float lz = cos_t;
     f3 d;
-    d.x = (lx * u.x + ly * v.x) + lz * n.x;
-    d.y = (lx * u.y + ly * v.y) + lz * n.y;
-    d.z = (lx * u.z + ly * v.z) + lz * n.z;
+    d.x = (lx * f.u.x + ly * f.v.x) + lz * f.n.x;
+    d.y = (lx * f.u.y + ly * f.v.y) + lz * f.n.y;
+    d.z = (lx * f.u.z + ly * f.v.z) + lz * f.n.z;
     dir_o = d;
     org_o.x = p.x + eps * d.x;
     org_o.y = p.y + eps * d.y;
     org_o.z = p.z + eps * d.z;
+}
+
+// The same with the frame of the (ray-facing) geometric normal n, the
+// SimpleShadingContext of a mesh without shading attributes.
+PRT_HD void diffuse_bounce(f3 org, f3 dir, float t, f3 n, float s1, float s2, f3& org_o, f3& dir_o)
+{
+    diffuse_bounce(org, dir, t, make_frame(n), s1, s2, org_o, dir_o);
 }
 
 // ---------------------------------------------------------------------------
@@ -282,8 +359,9 @@ struct TriCandidate {
     float t, v_num, w_num, det;
 };
 
-PRT_HD bool tri_test(const WatertightRay& w, f3 org, f3 dir, f3 p0, f3 p1, f3 p2, float tmin, float tmax,
-                     TriCandidate& c)
+// Scaled barycentrics U, V, W of p0, p1, p2 (the edge functions in the
+// ray's shear space); a zero one sends all three to double.
+PRT_HD void tri_edges(const WatertightRay& w, f3 org, f3 p0, f3 p1, f3 p2, float& U, float& V, float& W)
 {
     f3 a = sub3(p0, org), b = sub3(p1, org), cc = sub3(p2, org);
     float az_ = comp(a, w.kz), bz_ = comp(b, w.kz), cz_ = comp(cc, w.kz);
@@ -293,14 +371,21 @@ PRT_HD bool tri_test(const WatertightRay& w, f3 org, f3 dir, f3 p0, f3 p1, f3 p2
     float by = comp(b, w.ky) - w.sy * bz_;
     float cx = comp(cc, w.kx) - w.sx * cz_;
     float cy = comp(cc, w.ky) - w.sy * cz_;
-    float U = cx * by - cy * bx;
-    float V = ax * cy - ay * cx;
-    float W = bx * ay - by * ax;
+    U = cx * by - cy * bx;
+    V = ax * cy - ay * cx;
+    W = bx * ay - by * ax;
     if (U == 0.0f || V == 0.0f || W == 0.0f) {
         U = (float)((double)cx * (double)by - (double)cy * (double)bx);
         V = (float)((double)ax * (double)cy - (double)ay * (double)cx);
         W = (float)((double)bx * (double)ay - (double)by * (double)ax);
     }
+}
+
+PRT_HD bool tri_test(const WatertightRay& w, f3 org, f3 dir, f3 p0, f3 p1, f3 p2, float tmin, float tmax,
+                     TriCandidate& c)
+{
+    float U, V, W;
+    tri_edges(w, org, p0, p1, p2, U, V, W);
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
     float det = (U + V) + W;
     if (det == 0.0f) return false;
@@ -312,6 +397,18 @@ PRT_HD bool tri_test(const WatertightRay& w, f3 org, f3 dir, f3 p0, f3 p1, f3 p2
     c.w_num = W;
     c.det = det;
     return true;
+}
+
+// Hit coordinates (u, v) = (V/det, W/det) of a triangle that tri_test
+// accepted for this ray, recomputed from the same edge functions (so equal to
+// what tri_test's caller derives from the candidate).
+PRT_HD void tri_barycentrics(const WatertightRay& w, f3 org, f3 p0, f3 p1, f3 p2, float& u, float& v)
+{
+    float U, V, W;
+    tri_edges(w, org, p0, p1, p2, U, V, W);
+    const float det = (U + V) + W;
+    u = V / det;
+    v = W / det;
 }
 
 }  // namespace prt

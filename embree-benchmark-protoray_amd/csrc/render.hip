@@ -76,8 +76,13 @@ __device__ __forceinline__ void tri_vertices(const int32_t* idx, const float* po
     p2 = mk3(pos[3 * c], pos[3 * c + 1], pos[3 * c + 2]);
 }
 
-__device__ __forceinline__ void accum_add(float4* accum, int pixel, float c)
+// Adds a sample to a pixel, or (the "Fast" renderers) replaces the pixel.
+__device__ __forceinline__ void accum_add(float4* accum, int pixel, float c, bool overwrite)
 {
+    if (overwrite) {
+        accum[pixel] = make_float4(c, c, c, 1.0f);
+        return;
+    }
     float4 a = accum[pixel];
     accum[pixel] = make_float4(a.x + c, a.y + c, a.z + c, a.w + 1.0f);
 }
@@ -91,7 +96,8 @@ __global__ __launch_bounds__(kBlock) void k_shade_diffuse(const float4* rays_in,
                                                           int32_t* pix_out, uint32_t* rng_out,
                                                           uint32_t* out_count, float4* accum,
                                                           const int32_t* idx, const float* pos,
-                                                          float throughput, int final_depth)
+                                                          const float* normals, const float* texcoords,
+                                                          float throughput, int final_depth, bool overwrite)
 {
     const uint32_t count = *count_ptr;
     const int lane = threadIdx.x & 63;
@@ -111,22 +117,25 @@ __global__ __launch_bounds__(kBlock) void k_shade_diffuse(const float4* rays_in,
                 float4 h = hits[i];
                 if (h.x >= 0.0f) {
                     if (final_depth) {
-                        accum_add(accum, p, 0.0f);
+                        accum_add(accum, p, 0.0f, overwrite);
                     } else {
                         r0 = rays_in[2 * (size_t)i];
                         r1 = rays_in[2 * (size_t)i + 1];
                         f3 org = mk3(r0.x, r0.y, r0.z), dir = mk3(r1.x, r1.y, r1.z);
                         f3 p0, p1, p2;
-                        tri_vertices(idx, pos, __float_as_int(h.y), p0, p1, p2);
-                        f3 n = face_normal(p0, p1, p2, dir);
+                        const int prim = __float_as_int(h.y);
+                        tri_vertices(idx, pos, prim, p0, p1, p2);
+                        const Frame f = normals || texcoords
+                                            ? attribute_frame(idx, normals, texcoords, prim, p0, p1, p2, dir, h.z, h.w)
+                                            : make_frame(face_normal(p0, p1, p2, dir));
                         s = rng_in[i];
                         float s1 = sampler_next(s);
                         float s2 = sampler_next(s);
-                        diffuse_bounce(org, dir, h.x, n, s1, s2, no, nd);
+                        diffuse_bounce(org, dir, h.x, f, s1, s2, no, nd);
                         cont = true;
                     }
                 } else {
-                    accum_add(accum, p, throughput);
+                    accum_add(accum, p, throughput, overwrite);
                 }
             }
         }
@@ -147,12 +156,14 @@ __global__ __launch_bounds__(kBlock) void k_shade_diffuse(const float4* rays_in,
     }
 }
 
-// Primary visibility: (N + 1) / 2 with N the ray-facing geometric normal, 0.05
-// on a miss (render/primary_integrator_single.h:34-47).
+// Primary visibility: (N + 1) / 2 with N the shading normal (the ray-facing
+// geometric normal of a mesh without normals), 0.05 on a miss
+// (render/primary_integrator_single.h:34-47, render/primary_renderer_cuda.cu:59-90).
 __global__ __launch_bounds__(kBlock) void k_shade_primary(const float4* rays_in, const float4* hits,
                                                           const int32_t* pix_in, const uint32_t* count_ptr,
                                                           float4* accum, const int32_t* idx,
-                                                          const float* pos)
+                                                          const float* pos, const float* normals,
+                                                          const float* texcoords, bool overwrite)
 {
     const uint32_t count = *count_ptr;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
@@ -162,13 +173,21 @@ __global__ __launch_bounds__(kBlock) void k_shade_primary(const float4* rays_in,
         f3 c = mk3(0.05f, 0.05f, 0.05f);
         if (h.x >= 0.0f) {
             float4 r1 = rays_in[2 * (size_t)i + 1];
+            const f3 dir = mk3(r1.x, r1.y, r1.z);
             f3 p0, p1, p2;
-            tri_vertices(idx, pos, __float_as_int(h.y), p0, p1, p2);
-            f3 n = face_normal(p0, p1, p2, mk3(r1.x, r1.y, r1.z));
+            const int prim = __float_as_int(h.y);
+            tri_vertices(idx, pos, prim, p0, p1, p2);
+            const f3 n = normals || texcoords
+                             ? attribute_frame(idx, normals, texcoords, prim, p0, p1, p2, dir, h.z, h.w).n
+                             : face_normal(p0, p1, p2, dir);
             c = mk3((n.x + 1.0f) * 0.5f, (n.y + 1.0f) * 0.5f, (n.z + 1.0f) * 0.5f);
         }
-        float4 a = accum[p];
-        accum[p] = make_float4(a.x + c.x, a.y + c.y, a.z + c.z, a.w + 1.0f);
+        if (overwrite) {
+            accum[p] = make_float4(c.x, c.y, c.z, 1.0f);
+        } else {
+            float4 a = accum[p];
+            accum[p] = make_float4(a.x + c.x, a.y + c.y, a.z + c.z, a.w + 1.0f);
+        }
     }
 }
 
@@ -332,21 +351,22 @@ hipError_t launch_batch_advance(prt_hip_ctx* ctx)
     return hipGetLastError();
 }
 
-hipError_t launch_shade_diffuse(prt_hip_ctx* ctx, int depth, int max_depth, float throughput, float)
+hipError_t launch_shade_diffuse(prt_hip_ctx* ctx, int depth, int max_depth, float throughput, bool fast)
 {
     const int cur = depth & 1;
     k_shade_diffuse<<<persistent_blocks(ctx), kBlock, 0, ctx->stream>>>(
         ctx->d_rays[cur], ctx->d_hits, ctx->d_pix[cur], ctx->d_rng[cur], ctx->d_counters + depth,
         ctx->d_rays[1 - cur], ctx->d_pix[1 - cur], ctx->d_rng[1 - cur], ctx->d_counters + depth + 1,
-        ctx->d_accum, ctx->scene.d_idx, ctx->scene.d_pos, throughput, depth == max_depth ? 1 : 0);
+        ctx->d_accum, ctx->scene.d_idx, ctx->scene.d_pos, fast ? nullptr : ctx->scene.d_normals,
+        fast ? nullptr : ctx->scene.d_texcoords, throughput, depth == max_depth ? 1 : 0, fast);
     return hipGetLastError();
 }
 
-hipError_t launch_shade_primary(prt_hip_ctx* ctx)
+hipError_t launch_shade_primary(prt_hip_ctx* ctx, bool fast)
 {
     k_shade_primary<<<persistent_blocks(ctx), kBlock, 0, ctx->stream>>>(
         ctx->d_rays[0], ctx->d_hits, ctx->d_pix[0], ctx->d_counters, ctx->d_accum, ctx->scene.d_idx,
-        ctx->scene.d_pos);
+        ctx->scene.d_pos, fast ? nullptr : ctx->scene.d_normals, fast ? nullptr : ctx->scene.d_texcoords, fast);
     return hipGetLastError();
 }
 

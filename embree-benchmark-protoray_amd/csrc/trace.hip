@@ -111,6 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     const Bvh8Node* nodes = sc.nodes;
     const float* tris = sc.tris;
     const int32_t* prim_ids = sc.prim_ids;
+    int32_t scene_id = 0;  // batch mode: the lane's scene (for its shading attributes)
     float* sample_out = nullptr;
     // RGB throughput (RGB and adjoint modes); adjoint: per-channel product of
     // the path's nonzero albedos and count of zero albedos from the primal
@@ -155,6 +156,28 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
         node_t = near;
         pend = 0;
         tracing = true;
+    };
+
+    // Shading frame of the current ray's closest hit (triangle p0, p1, p2):
+    // the full ShadingContext when the scene has per-vertex attributes (the
+    // hit coordinates are recomputed from the winner's edge functions), else
+    // the frame of the ray-facing geometric normal.
+    auto hit_frame = [&](f3 p0, f3 p1, f3 p2) -> Frame {
+        const int32_t* aidx = sc.idx;
+        const float* an = pl.fast ? nullptr : sc.normals;
+        const float* at = pl.fast ? nullptr : sc.texcoords;
+        if (kMode == kModeBatch) {
+            const SceneDevice& js = bl.scenes[scene_id];
+            aidx = js.idx;
+            an = js.normals;
+            at = js.texcoords;
+        }
+        if (an != nullptr || at != nullptr) {
+            float hu, hv;
+            tri_barycentrics(wr, o, p0, p1, p2, hu, hv);
+            return attribute_frame(aidx, an, at, best_prim, p0, p1, p2, d, hu, hv);
+        }
+        return make_frame(face_normal(p0, p1, p2, d));
     };
 
     for (;;) {
@@ -206,20 +229,20 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                     done = true;
                 }
                 if (done) {
-                    const float4 a = *acc;
+                    const float4 a = pl.fast ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : *acc;
                     *acc = make_float4(a.x + lsum, a.y + lsum, a.z + lsum, a.w + 1.0f);
                     active = false;
                     break;
                 }
                 lw *= albedo;
-                const f3 n = face_normal(p0, p1, p2, d);
+                const Frame fr = hit_frame(p0, p1, p2);
                 const float s1 = sampler_next(rng);
                 const float s2 = sampler_next(rng);
                 const float s3 = sampler_next(rng);
                 const float s4 = sampler_next(rng);
                 f3 so, sd;
-                diffuse_bounce(o, d, t, n, s1, s2, so, sd);
-                diffuse_bounce(o, d, t, n, s3, s4, eo, ed);
+                diffuse_bounce(o, d, t, fr, s1, s2, so, sd);
+                diffuse_bounce(o, d, t, fr, s3, s4, eo, ed);
                 ++depth;
                 shadow = true;
                 start_ray(so, sd, 0.0f, 3.402823466e38f);
@@ -228,10 +251,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             if (kMode == kModePrimary) {
                 f3 c = mk3(0.05f, 0.05f, 0.05f);
                 if (best_prim >= 0) {
-                    const f3 n = face_normal(p0, p1, p2, d);
+                    const f3 n = hit_frame(p0, p1, p2).n;
                     c = mk3((n.x + 1.0f) * 0.5f, (n.y + 1.0f) * 0.5f, (n.z + 1.0f) * 0.5f);
                 }
-                const float4 a = *acc;
+                const float4 a = pl.fast ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : *acc;
                 *acc = make_float4(a.x + c.x, a.y + c.y, a.z + c.z, a.w + 1.0f);
                 active = false;
                 break;
@@ -265,7 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                     const float4 a = *acc;
                     *acc = make_float4(a.x + c, a.y + cg, a.z + cb, a.w + 1.0f);
                 } else {
-                    const float4 a = *acc;
+                    const float4 a = pl.fast ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : *acc;
                     *acc = make_float4(a.x + c, a.y + c, a.z + c, a.w + 1.0f);
                 }
                 active = false;
@@ -300,11 +323,11 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             } else {
                 lw *= albedo;
             }
-            const f3 n = face_normal(p0, p1, p2, d);
+            const Frame fr = hit_frame(p0, p1, p2);
             const float s1 = sampler_next(rng);
             const float s2 = sampler_next(rng);
             f3 no, nd;
-            diffuse_bounce(o, d, t, n, s1, s2, no, nd);
+            diffuse_bounce(o, d, t, fr, s1, s2, no, nd);
             ++depth;
             start_ray(no, nd, 0.0f, 3.402823466e38f);
         } while (0);
@@ -354,6 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                         if (x < job.width && y < job.height) {
                             pixel = x + y * job.width;
                             const SceneDevice jsc = bl.scenes[job.scene];
+                            scene_id = job.scene;
                             nodes = jsc.nodes;
                             tris = jsc.tris;
                             prim_ids = jsc.prim_ids;

@@ -234,6 +234,50 @@ def scene_spheres(seed: int = 1002, count: int = 256, subdiv: int = 4) -> Mesh:
     return merge(parts)
 
 
+def vertex_normals(idx: np.ndarray, pos: np.ndarray) -> np.ndarray:
+    """Smooth-shading vertex normals: the normalised sum of the area-weighted
+    normals of the triangles around each vertex."""
+    p = pos.astype(np.float64)
+    fn = np.cross(p[idx[:, 1]] - p[idx[:, 0]], p[idx[:, 2]] - p[idx[:, 0]])
+    vn = np.zeros_like(p)
+    for k in range(3):
+        np.add.at(vn, idx[:, k], fn)
+    ln = np.linalg.norm(vn, axis=1, keepdims=True)
+    return (vn / np.where(ln > 0, ln, 1.0)).astype(np.float32)
+
+
+def scene_smooth(seed: int = 1006, cells: int = 32, spheres: int = 24, subdiv: int = 3) -> Mesh:
+    """A mesh with per-vertex normals and texture coordinates for the full
+    shading context: a heightfield (heights U[0, 0.3]) on [-1,1]^2 with
+    smooth vertex normals and planar texture coordinates (x, z); icospheres
+    above it with their exact normals and latitude/longitude texture
+    coordinates; a wall quad at z = 1 whose vertex normals point against its
+    geometric normal and whose texture coordinates are all equal (a
+    degenerate mapping)."""
+    rng = np.random.default_rng(seed)
+    hidx, hpos = heightfield(cells, rng, 0.3)
+    parts = [(hidx, hpos, vertex_normals(hidx, hpos), np.stack([hpos[:, 0], hpos[:, 2]], 1))]
+    f, v = icosphere(subdiv)
+    lon = (np.arctan2(v[:, 2], v[:, 0]) / (2 * math.pi) + 0.5).astype(np.float32)
+    lat = (np.arccos(np.clip(v[:, 1], -1.0, 1.0)) / math.pi).astype(np.float32)
+    for _ in range(spheres):
+        c = rng.uniform([-0.8, 0.35, -0.8], [0.8, 0.9, 0.8])
+        r = rng.uniform(0.08, 0.25)
+        parts.append((f, (v * r + c).astype(np.float32), v.astype(np.float32), np.stack([lon, lat], 1)))
+    widx, wpos = quad((-1, 0, 1), (1, 0, 1), (1, 1.2, 1), (-1, 1.2, 1))
+    wn = np.cross(wpos[1] - wpos[0], wpos[2] - wpos[0])
+    parts.append((widx, wpos, np.tile(-wn / np.linalg.norm(wn), (4, 1)).astype(np.float32),
+                  np.full((4, 2), 0.25, np.float32)))
+    m = merge([(i, p) for i, p, _, _ in parts])
+    m.normals = np.concatenate([n for _, _, n, _ in parts]).astype(np.float32)
+    m.texcoords = np.concatenate([t for _, _, _, t in parts]).astype(np.float32)
+    return m
+
+
+# a view of scene_smooth from z = -2.6 towards +z: spheres, heightfield, wall
+SMOOTH_VIEW = {"pos": [0.0, 1.6, -2.6], "angle_x": 0.45, "angle_y": math.pi, "fov_y": 0.9}
+
+
 def scene_convex_sphere(subdiv: int = 3) -> Mesh:
     f, v = icosphere(subdiv)
     return Mesh(f, v.astype(np.float32))

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PRT_HIP_ABI_VERSION 2
+#define PRT_HIP_ABI_VERSION 3
 
 typedef struct prt_hip_ctx prt_hip_ctx;
 
@@ -120,7 +120,13 @@ typedef struct prt_trace_counters {
 enum {
     PRT_RENDERER_PRIMARY = 0,   /* "primaryHip": (N+1)/2 on hit, 0.05 on miss */
     PRT_RENDERER_DIFFUSE = 1,   /* "diffuseHip": diffuse path tracing         */
-    PRT_RENDERER_DIFFUSE2 = 2   /* "diffuse2Hip": with next-event shadow rays  */
+    PRT_RENDERER_DIFFUSE2 = 2,  /* "diffuse2Hip": with next-event shadow rays  */
+    /* modifier bit of the "Fast" renderers ("primaryFast", "diffuseFast",
+     * "diffuse2Fast", render/device_cuda.cpp:79-90): SimpleShadingContext
+     * (frame of the geometric normal, attributes ignored) and each pass
+     * overwrites the accumulator instead of adding to it (isAccum = false,
+     * render/diffuse_renderer_cuda.cu:109-118) */
+    PRT_RENDERER_FAST = 0x100
 };
 
 /* --- context ------------------------------------------------------------ */
@@ -136,6 +142,15 @@ int         prt_hip_device_name(prt_hip_ctx* ctx, char* buf, int buf_len);
  * the strided .mesh vertex attributes, render/device_cuda.cpp:135-142). */
 int prt_hip_scene_upload(prt_hip_ctx* ctx, const int32_t* idx3, int64_t tri_count,
                          const float* pos3, int64_t vertex_count);
+/* Per-vertex shading attributes of the uploaded scene (the optional normals
+ * and texcoords that DeviceCuda::initScene uploads, render/device_cuda.cpp:
+ * 144-160): normals3 = 3 floats per vertex, texcoords2 = 2 floats per vertex,
+ * in the vertex order of the upload; NULL leaves the mesh without that
+ * attribute.  prt_hip_scene_upload removes both.  With them the primary,
+ * diffuse and diffuse2 renderers shade with the full ShadingContext
+ * (geometry/triangle_mesh_cuda.cuh:41-130): interpolated vertex normal and a
+ * frame aligned with dp/du; the PRT_RENDERER_FAST renderers ignore them. */
+int prt_hip_scene_attributes(prt_hip_ctx* ctx, const float* normals3, const float* texcoords2);
 
 /* --- acceleration structure (OptixIntersectorStreamCuda ctor) ------------ */
 /* LBVH on the GPU: Morton codes, LSD radix sort, Karras radix tree, bottom-up
@@ -275,6 +290,9 @@ typedef struct prt_batch_tuning {
 
 int prt_hip_batch_scene_add(prt_hip_ctx* ctx, const int32_t* idx3, int64_t tri_count, const float* pos3,
                             int64_t vertex_count, prt_build_stats* out_stats, int32_t* out_scene);
+/* Shading attributes of batch scene `scene`, as prt_hip_scene_attributes. */
+int prt_hip_batch_scene_attributes(prt_hip_ctx* ctx, int32_t scene, const float* normals3,
+                                   const float* texcoords2);
 int prt_hip_batch_reset(prt_hip_ctx* ctx);
 /* Allocates the accumulators (zeroed) and sample planes of the jobs. */
 int prt_hip_batch_set_jobs(prt_hip_ctx* ctx, const prt_batch_job* jobs, int n_jobs);

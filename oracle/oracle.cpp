@@ -141,6 +141,11 @@ struct Scene {
     std::vector<Tri> tris;
     std::vector<int> order;   // leaf order -> triangle id
     std::vector<Node> nodes;
+    // vertex indices and the optional per-vertex shading attributes
+    std::vector<int32_t> idx;     // 3 per triangle
+    std::vector<V3> normals;      // empty: no normals
+    std::vector<float> texcoords; // 2 per vertex; empty: none
+    int64_t vertex_count = 0;
 };
 
 void tri_bounds(const Tri& t, float lo[3], float hi[3])
@@ -644,14 +649,82 @@ V3 facing_normal(const Tri& tr, V3 dir)
     return ng;
 }
 
+V3 neg(V3 a) { return v3(-a.x, -a.y, -a.z); }
+
+struct Frame {
+    V3 u, v, n;
+};
+
+// makeFrame (math/basis3.h:128-136)
+Frame make_frame(V3 n)
+{
+    V3 u0 = v3(0.0f, n.z, -n.y), u1 = v3(-n.z, 0.0f, n.x);
+    Frame f;
+    f.u = normalized(dot(u0, u0) > dot(u1, u1) ? u0 : u1);
+    f.v = normalized(cross(n, f.u));
+    f.n = n;
+    return f;
+}
+
+// Shading frame of a hit on triangle `prim` at hit coordinates (hu, hv)
+// (p = (1-hu-hv) p0 + hu p1 + hv p2) for a ray along dir.  fast or a mesh
+// without attributes: SimpleShadingContext, makeFrame of the ray-facing
+// geometric normal (geometry/triangle_mesh.cpp:248-273).  Otherwise the
+// ShadingContext of TriangleMesh::postIntersect (geometry/triangle_mesh.cpp:
+// 59-153): N = normalize(b0 n0 + b1 n1 + b2 n2) turned to the side of Ng
+// (Ng without normals), both flipped if Ng faces along the ray; with texture
+// coordinates U = normalize(dp/du), V = normalize(N x U), U = V x N, unless
+// the mapping's determinant is 0 or N x U vanishes (then makeFrame(N)).
+Frame hit_frame(const Scene& sc, int prim, float hu, float hv, V3 dir, bool fast)
+{
+    const Tri& tr = sc.tris[prim];
+    const bool hn = !sc.normals.empty() && !fast, ht = !sc.texcoords.empty() && !fast;
+    if (!hn && !ht) return make_frame(facing_normal(tr, dir));
+    const int i0 = sc.idx[3 * (size_t)prim], i1 = sc.idx[3 * (size_t)prim + 1], i2 = sc.idx[3 * (size_t)prim + 2];
+    const float b1 = hu, b2 = hv;
+    const float b0 = 1.0f - b1 - b2;
+    const V3 dp1 = tr.p1 - tr.p0, dp2 = tr.p2 - tr.p0;
+    const V3 ng = normalized(cross(dp1, dp2));
+    V3 n = ng;
+    if (hn) {
+        const V3 n0 = sc.normals[i0], n1 = sc.normals[i1], n2 = sc.normals[i2];
+        n = normalized(v3((b0 * n0.x + b1 * n1.x) + b2 * n2.x, (b0 * n0.y + b1 * n1.y) + b2 * n2.y,
+                          (b0 * n0.z + b1 * n1.z) + b2 * n2.z));
+        if (dot(n, ng) < 0.0f) n = neg(n);
+    }
+    if (dot(ng, dir) > 0.0f) n = neg(n);
+    if (ht) {
+        const float* t0 = &sc.texcoords[2 * (size_t)i0];
+        const float* t1 = &sc.texcoords[2 * (size_t)i1];
+        const float* t2 = &sc.texcoords[2 * (size_t)i2];
+        const float du1 = t1[0] - t0[0], du2 = t2[0] - t0[0];
+        const float dv1 = t1[1] - t0[1], dv2 = t2[1] - t0[1];
+        const float det = du1 * dv2 - dv1 * du2;
+        if (det != 0.0f) {
+            const float inv = 1.0f / det;
+            V3 dpdu = v3((dv2 * dp1.x - dv1 * dp2.x) * inv, (dv2 * dp1.y - dv1 * dp2.y) * inv,
+                         (dv2 * dp1.z - dv1 * dp2.z) * inv);
+            Frame f;
+            f.n = n;
+            f.u = normalized(dpdu);
+            f.v = cross(n, f.u);
+            if (dot(f.v, f.v) > 0.0f) {
+                f.v = normalized(f.v);
+                f.u = cross(f.v, n);
+                return f;
+            }
+        }
+    }
+    return make_frame(n);
+}
+
+// Diffuse bounce in frame f (render/diffuse_integrator_single.h:55-62,
+// core/ray.h:71-76, sampling/shape_sampler.h:50-74).
 // numerics 0: kernel fp32 (polynomial sin/cos); 1: reference (libm sinf/cosf)
-void bounce(V3 org, V3 dir, float t, V3 n, float s1, float s2, int numerics, V3& no, V3& nd)
+void bounce(V3 org, V3 dir, float t, const Frame& f, float s1, float s2, int numerics, V3& no, V3& nd)
 {
     V3 p = v3(org.x + t * dir.x, org.y + t * dir.y, org.z + t * dir.z);
     float eps = std::max(t, std::max(std::max(std::fabs(p.x), std::fabs(p.y)), std::fabs(p.z))) * 0x1.fp-18f;
-    V3 u0 = v3(0.0f, n.z, -n.y), u1 = v3(-n.z, 0.0f, n.x);
-    V3 u = normalized(dot(u0, u0) > dot(u1, u1) ? u0 : u1);
-    V3 v = normalized(cross(n, u));
     float ct = std::sqrt(s1), st = std::sqrt(1.0f - s1);
     float sp, cp;
     if (numerics == 0) {
@@ -662,23 +735,29 @@ void bounce(V3 org, V3 dir, float t, V3 n, float s1, float s2, int numerics, V3&
         cp = std::cos(phi);
     }
     float lx = cp * st, ly = sp * st, lz = ct;
+    const V3 u = f.u, v = f.v, n = f.n;
     nd = v3((lx * u.x + ly * v.x) + lz * n.x, (lx * u.y + ly * v.y) + lz * n.y, (lx * u.z + ly * v.z) + lz * n.z);
     no = v3(p.x + eps * nd.x, p.y + eps * nd.y, p.z + eps * nd.z);
 }
 
+void bounce(V3 org, V3 dir, float t, V3 n, float s1, float s2, int numerics, V3& no, V3& nd)
+{
+    bounce(org, dir, t, make_frame(n), s1, s2, numerics, no, nd);
+}
+
 struct HitAny {
     int prim;
-    float t;
+    float t, u, v;
 };
 
 HitAny closest_any(const Scene& sc, V3 o, V3 d, int numerics)
 {
     if (numerics == 0) {
         Closest32 c = closest_fp32(sc, o, d, 0.0f, 3.402823466e38f);
-        return HitAny{c.prim, c.t};
+        return HitAny{c.prim, c.t, c.u, c.v};
     }
     Closest64 c = closest_fp64(sc, o, d, 0.0f, 3.402823466e38f, false);
-    return HitAny{c.prim, (float)c.t};
+    return HitAny{c.prim, (float)c.t, (float)c.u, (float)c.v};
 }
 
 bool occluded_any(const Scene& sc, V3 o, V3 d, int numerics)
@@ -793,9 +872,35 @@ void* orc_scene_create(const int32_t* idx, int64_t tri_count, const float* pos, 
         const float* c = pos + 3 * (size_t)idx[3 * i + 2];
         sc->tris[i] = Tri{v3(a[0], a[1], a[2]), v3(b[0], b[1], b[2]), v3(c[0], c[1], c[2])};
     }
-    (void)vertex_count;
+    sc->idx.assign(idx, idx + 3 * tri_count);
+    sc->vertex_count = vertex_count;
     build_bvh(*sc);
     return sc;
+}
+
+// Per-vertex shading attributes: normals (3 floats per vertex) and texture
+// coordinates (2 floats per vertex); null removes the attribute.
+void orc_scene_attributes(void* scp, const float* normals, const float* texcoords)
+{
+    Scene& sc = *static_cast<Scene*>(scp);
+    sc.normals.clear();
+    sc.texcoords.clear();
+    if (normals)
+        for (int64_t i = 0; i < sc.vertex_count; ++i)
+            sc.normals.push_back(v3(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]));
+    if (texcoords) sc.texcoords.assign(texcoords, texcoords + 2 * sc.vertex_count);
+}
+
+// Shading frame of hit (prim, u, v) for a ray along dir: out = U, V, N.
+void orc_shading_frame(void* scp, int prim, float u, float v, const float* dir, int fast, float* out)
+{
+    const Frame f = hit_frame(*static_cast<Scene*>(scp), prim, u, v, v3(dir[0], dir[1], dir[2]), fast != 0);
+    const V3 c[3] = {f.u, f.v, f.n};
+    for (int k = 0; k < 3; ++k) {
+        out[3 * k] = c[k].x;
+        out[3 * k + 1] = c[k].y;
+        out[3 * k + 2] = c[k].z;
+    }
 }
 
 void orc_scene_destroy(void* sc) { delete static_cast<Scene*>(sc); }
@@ -925,12 +1030,17 @@ void orc_intersect_fp32(void* scp, const float* rays, int64_t n, float* hits, in
 // PrimaryIntegratorSingle), passes pass0 .. pass0+spp-1, 1 sample per pass.
 // accum: 4 floats per pixel (sum r, g, b, weight); sumsq: per-pixel sum of the
 // squared grey value (for the Monte Carlo variance); rays_out: intersect calls.
-// renderer 0 = primary, 1 = diffuse, 2 = diffuse2 (next-event shadow rays);
+// renderer 0 = primary, 1 = diffuse, 2 = diffuse2 (next-event shadow rays),
+// plus 0x100 for the "Fast" variant (SimpleShadingContext, each pass
+// overwrites the pixel: render/device_cuda.cpp:79-90,
+// render/diffuse_renderer_cuda.cu:109-118);
 // numerics 0 = kernel fp32, 1 = reference fp64.
 int orc_render(void* scp, const float* cam, int w, int h, int pass0, int spp, int max_depth, float albedo,
                int renderer, int numerics, int threads, float* accum, float* sumsq, int64_t* rays_out)
 {
     const Scene& sc = *static_cast<Scene*>(scp);
+    const bool fast = (renderer & 0x100) != 0;
+    renderer &= 0xff;
     Cam c = cam_from(cam);
     float inv_w = 1.0f / (float)w, inv_h = 1.0f / (float)h;
     int nt = default_threads(threads);
@@ -953,7 +1063,7 @@ int orc_render(void* scp, const float* cam, int w, int h, int pass0, int spp, in
                         HitAny hit = closest_any(sc, org, dir, numerics);
                         ++local_rays;
                         if (hit.prim >= 0) {
-                            V3 n = facing_normal(sc.tris[hit.prim], dir);
+                            V3 n = hit_frame(sc, hit.prim, hit.u, hit.v, dir, fast).n;
                             col[0] = (n.x + 1.0f) * 0.5f;
                             col[1] = (n.y + 1.0f) * 0.5f;
                             col[2] = (n.z + 1.0f) * 0.5f;
@@ -973,15 +1083,15 @@ int orc_render(void* scp, const float* cam, int w, int h, int pass0, int spp, in
                             }
                             if (depth == max_depth) break;
                             lw *= albedo;
-                            V3 n = facing_normal(sc.tris[hit.prim], dir);
+                            Frame fr = hit_frame(sc, hit.prim, hit.u, hit.v, dir, fast);
                             float s1 = s.next(), s2 = s.next();
                             V3 so, sd;
-                            bounce(org, dir, hit.t, n, s1, s2, numerics, so, sd);
+                            bounce(org, dir, hit.t, fr, s1, s2, numerics, so, sd);
                             ++local_rays;
                             if (!occluded_any(sc, so, sd, numerics)) L += lw * 0.5f;
                             float s3 = s.next(), s4 = s.next();
                             V3 no, nd;
-                            bounce(org, dir, hit.t, n, s3, s4, numerics, no, nd);
+                            bounce(org, dir, hit.t, fr, s3, s4, numerics, no, nd);
                             org = no;
                             dir = nd;
                             ++depth;
@@ -996,10 +1106,10 @@ int orc_render(void* scp, const float* cam, int w, int h, int pass0, int spp, in
                             ++local_rays;
                             if (hit.prim < 0 || depth == max_depth) break;
                             lw *= albedo;
-                            V3 n = facing_normal(sc.tris[hit.prim], dir);
+                            Frame fr = hit_frame(sc, hit.prim, hit.u, hit.v, dir, fast);
                             float s1 = s.next(), s2 = s.next();
                             V3 no, nd;
-                            bounce(org, dir, hit.t, n, s1, s2, numerics, no, nd);
+                            bounce(org, dir, hit.t, fr, s1, s2, numerics, no, nd);
                             org = no;
                             dir = nd;
                             ++depth;
@@ -1008,6 +1118,7 @@ int orc_render(void* scp, const float* cam, int w, int h, int pass0, int spp, in
                         col[0] = col[1] = col[2] = L;
                     }
                     float* a = accum + 4 * (size_t)pid;
+                    if (fast) a[0] = a[1] = a[2] = a[3] = 0.0f;  // isAccum = false: overwrite
                     a[0] = a[0] + col[0];
                     a[1] = a[1] + col[1];
                     a[2] = a[2] + col[2];
@@ -1164,10 +1275,10 @@ void* orc_paths_record(void* scp, const float* cam, int w, int h, int pass0, int
                 ++local;
                 if (hit.prim < 0 || depth == max_depth) break;
                 hl.push_back(hit.prim);
-                V3 n = facing_normal(sc.tris[hit.prim], dir);
+                Frame fr = hit_frame(sc, hit.prim, hit.u, hit.v, dir, false);
                 float s1 = s.next(), s2 = s.next();
                 V3 no, nd;
-                bounce(org, dir, hit.t, n, s1, s2, 0, no, nd);
+                bounce(org, dir, hit.t, fr, s1, s2, 0, no, nd);
                 org = no;
                 dir = nd;
                 ++depth;

@@ -37,6 +37,8 @@ def lib() -> ctypes.CDLL:
         L.orc_scene_create.argtypes = [P, i64, P, i64]
         L.orc_scene_create.restype = P
         L.orc_scene_destroy.argtypes = [P]
+        L.orc_scene_attributes.argtypes = [P, P, P]
+        L.orc_shading_frame.argtypes = [P, i, f, f, P, i, P]
         L.orc_intersect_fp64.argtypes = [P, P, i64, P, P, P, P, P, P, P, i]
         L.orc_ray_edge_distance.argtypes = [P, P, P, i64, P]
         L.orc_intersect_fp32.argtypes = [P, P, i64, P, i]
@@ -126,6 +128,23 @@ class Scene:
         if self.h:
             lib().orc_scene_destroy(self.h)
             self.h = None
+
+    def set_attributes(self, normals: np.ndarray | None = None, texcoords: np.ndarray | None = None) -> None:
+        """Per-vertex normals (V x 3) and texture coordinates (V x 2); None removes one."""
+        self.normals = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        self.texcoords = None if texcoords is None else np.ascontiguousarray(texcoords, np.float32).reshape(-1, 2)
+        for a in (self.normals, self.texcoords):
+            if a is not None and a.shape[0] != self.pos.shape[0]:
+                raise ValueError("one attribute per vertex expected")
+        lib().orc_scene_attributes(self.h, None if self.normals is None else _p(self.normals),
+                                   None if self.texcoords is None else _p(self.texcoords))
+
+    def shading_frame(self, prim: int, u: float, v: float, direction, fast: bool = False) -> np.ndarray:
+        """Shading frame (rows U, V, N) of hit (prim, u, v) for a ray along direction."""
+        d = np.ascontiguousarray(direction, np.float32)
+        out = np.zeros((3, 3), np.float32)
+        lib().orc_shading_frame(self.h, int(prim), float(u), float(v), _p(d), int(fast), _p(out))
+        return out
 
     def __del__(self):
         try:
