@@ -137,6 +137,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "prt_hip_set_pass": ([P, ctypes.c_int], ctypes.c_int),
         "prt_hip_set_profiling": ([P, ctypes.c_int], ctypes.c_int),
         "prt_hip_set_pipeline": ([P, ctypes.c_int], ctypes.c_int),
+        "prt_hip_set_bvh_collapse": ([P, ctypes.c_int, ctypes.c_float], ctypes.c_int),
         "prt_hip_accum_read": ([P, P], ctypes.c_int),
         "prt_hip_frame_read_bgr8": ([P, P], ctypes.c_int),
         "prt_hip_debug_raygen": ([P, ctypes.c_int, P, P], ctypes.c_int),
@@ -281,6 +282,13 @@ class HipDevice:
         n, t = _attributes(normals, texcoords, self.vertex_count)
         self._check(self.lib.prt_hip_scene_attributes(self.ctx, _ptr_or_null(n), _ptr_or_null(t)),
                     "scene_attributes")
+
+    def set_bvh_collapse(self, method: str = "sah", prim_cost: float = 1.0) -> None:
+        """BVH8 collapse of later builds: 'sah' (surface-area cost optimum) or 'greedy'."""
+        ids = {"greedy": 0, "sah": 1}
+        if method not in ids:
+            raise ValueError("method must be 'sah' or 'greedy'")
+        self._check(self.lib.prt_hip_set_bvh_collapse(self.ctx, ids[method], prim_cost), "set_bvh_collapse")
 
     def build(self) -> dict:
         st = PrtBuildStats()

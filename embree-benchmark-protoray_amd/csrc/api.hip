@@ -290,6 +290,16 @@ int prt_hip_scene_attributes(prt_hip_ctx* ctx, const float* normals3, const floa
     return set_attributes(ctx, ctx->scene, normals3, texcoords2);
 }
 
+int prt_hip_set_bvh_collapse(prt_hip_ctx* ctx, int method, float prim_cost)
+{
+    if (!ctx) return 1;
+    if ((method != PRT_COLLAPSE_GREEDY && method != PRT_COLLAPSE_SAH) || !(prim_cost > 0.0f && prim_cost < 100.0f))
+        return set_error(ctx, "set_bvh_collapse: unknown method or prim_cost outside (0, 100)");
+    ctx->bvh_collapse = method;
+    ctx->bvh_prim_cost = prim_cost;
+    return 0;
+}
+
 int prt_hip_bvh_build(prt_hip_ctx* ctx, prt_build_stats* out_stats)
 {
     if (use_device(ctx)) return 1;

@@ -9,10 +9,15 @@
 //      BVHs, Octrees, and k-d Trees", HPG 2012), equal keys split by index
 //   5. bottom-up boxes with one arrival counter per internal node: inside
 //      4096-leaf tiles in LDS, across tiles with device-scope atomics
-//   6. top-down collapse to BVH8 one level per launch: the widest-area
-//      internal child is opened until 8 slots are used; subtrees of at most
-//      kLeafMax triangles become leaf slots.  Node and record offsets come
-//      from exclusive scans, so the layout is deterministic.
+//   6. top-down collapse to BVH8 one level per launch.  Which binary nodes
+//      become the (at most 8) slots of a wide node follows a surface-area
+//      cost optimum computed bottom-up during the refit (S. Ylitie, T.
+//      Karras, S. Laine, "Efficient Incoherent Ray Traversal on GPUs Through
+//      Compressed Wide BVHs", HPG 2017, section 4.1); subtrees of at most
+//      kLeafMax triangles become leaf slots.  (The greedy alternative opens
+//      the widest-area internal child until 8 slots are used.)  Node and
+//      record offsets come from exclusive scans, so the layout is
+//      deterministic.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -49,6 +54,84 @@ __device__ __forceinline__ Box tri_box(const int32_t* idx, const float* pos, int
     r.hi[1] = fmaxf(fmaxf(a.y, b.y), c.y);
     r.hi[2] = fmaxf(fmaxf(a.z, b.z), c.z);
     return r;
+}
+
+__device__ __forceinline__ float box_half_area(const Box& b)
+{
+    const float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// ---------------------------------------------------------------- collapse cost
+// cost(n, i), i = 1..7: the least surface-area cost of covering the subtree of
+// binary node n with at most i slots of the parent wide node.  A slot that is
+// a subtree of at most kLeafMax triangles is a leaf (area x triangles x
+// c_prim); a larger one is a wide node (area x c_node plus the best split of
+// its 8 slots between its two children).  Per internal node the array holds
+// the 7 costs and one word of choices: bits 0-2 the left child's share of
+// the wide node's 8 slots, bits 3(i-1)..3(i-1)+2 for i = 2..7 the left share
+// of i slots (0: cost(n, i-1) is no worse, use fewer).
+constexpr int kDpWords = 8;
+
+struct DpCost {
+    float c[7];
+};
+
+__device__ __forceinline__ DpCost dp_leaf(float area, float c_prim)
+{
+    DpCost r;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) r.c[i] = area * c_prim;
+    return r;
+}
+
+__device__ __forceinline__ DpCost dp_combine(const DpCost& l, const DpCost& r, float area, int size, float c_node,
+                                             float c_prim, uint32_t& choice)
+{
+    float dist[9];
+    int dk[9];
+#pragma unroll
+    for (int j = 2; j <= 8; ++j) {
+        float best = __builtin_inff();
+        int bk = 1;
+#pragma unroll
+        for (int k = 1; k < j; ++k) {
+            if (k > 7 || j - k > 7) continue;
+            const float v = l.c[k - 1] + r.c[j - k - 1];
+            if (v < best) {
+                best = v;
+                bk = k;
+            }
+        }
+        dist[j] = best;
+        dk[j] = bk;
+    }
+    DpCost out;
+    out.c[0] = size <= kLeafMax ? area * (float)size * c_prim : area * c_node + dist[8];
+    choice = (uint32_t)dk[8];
+#pragma unroll
+    for (int i = 2; i <= 7; ++i) {
+        if (dist[i] < out.c[i - 2]) {
+            out.c[i - 1] = dist[i];
+            choice |= (uint32_t)dk[i] << (3 * (i - 1));
+        } else {
+            out.c[i - 1] = out.c[i - 2];
+        }
+    }
+    return out;
+}
+
+// Written and read as agent-scope atomics: in the refit's first phase a
+// sibling's costs come from another thread of the workgroup (ordered by the
+// LDS arrival counter and workgroup fences), in the second from any
+// workgroup (device-scope fences).
+__device__ __forceinline__ void dp_store(uint32_t* dp, int32_t node, const DpCost& c, uint32_t choice)
+{
+    uint32_t* d = dp + (int64_t)kDpWords * node;
+#pragma unroll
+    for (int i = 0; i < 7; ++i)
+        __hip_atomic_store(d + i, __float_as_uint(c.c[i]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(d + 7, choice, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------- bounds
@@ -315,6 +398,17 @@ __device__ __forceinline__ Box child_box(int32_t c, const uint32_t* sorted, cons
     return b;
 }
 
+__device__ __forceinline__ DpCost child_cost(int32_t c, const Box& cb, const uint32_t* dp, float c_prim)
+{
+    if (c < 0) return dp_leaf(box_half_area(cb), c_prim);
+    DpCost r;
+#pragma unroll
+    for (int i = 0; i < 7; ++i)
+        r.c[i] = __uint_as_float(__hip_atomic_load(dp + (int64_t)kDpWords * c + i, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT));
+    return r;
+}
+
 // Two-phase refit.  Phase A: one workgroup per tile of kRefitTile sorted
 // leaves finishes every internal node whose leaf range lies inside the tile.
 // A Karras node's index is one end of its range, so those nodes have indices
@@ -328,7 +422,8 @@ constexpr int kRefitThreads = 1024;
 __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
     int64_t n, const uint32_t* sorted, const int32_t* idx, const float* pos, const int32_t* left,
     const int32_t* right, const int32_t* first, const int32_t* last, const int32_t* iparent,
-    const int32_t* lparent, float* ibox, int32_t* pending, uint32_t* pending_count)
+    const int32_t* lparent, float* ibox, int32_t* pending, uint32_t* pending_count, uint32_t* dp, float c_node,
+    float c_prim)
 {
     __shared__ uint32_t flag[kRefitTile];
     __shared__ float lbox[kRefitTile][6];
@@ -341,6 +436,7 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
         if (j > hi) break;
         Box b = tri_box(idx, pos, (int)sorted[j]);
         int32_t cur = ~(int32_t)j;
+        DpCost cur_cost = dp_leaf(box_half_area(b), c_prim);
         int32_t p = lparent[j];
         while (p >= 0) {
             if (first[p] < lo || last[p] > hi) {
@@ -352,7 +448,8 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
             __threadfence_block();
             if (atomicAdd(&flag[s], 1u) == 0u) break;
             __threadfence_block();
-            const int32_t sib = left[p] == cur ? right[p] : left[p];
+            const bool cur_left = left[p] == cur;
+            const int32_t sib = cur_left ? right[p] : left[p];
             Box o;
             if (sib < 0) {
                 o = tri_box(idx, pos, (int)sorted[~sib]);
@@ -370,6 +467,15 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
                 ibox[6 * (int64_t)p + a] = b.lo[a];
                 ibox[6 * (int64_t)p + 3 + a] = b.hi[a];
             }
+            if (dp) {
+                const DpCost sib_cost = child_cost(sib, o, dp, c_prim);
+                uint32_t choice;
+                cur_cost = cur_left ? dp_combine(cur_cost, sib_cost, box_half_area(b), last[p] - first[p] + 1,
+                                                 c_node, c_prim, choice)
+                                    : dp_combine(sib_cost, cur_cost, box_half_area(b), last[p] - first[p] + 1,
+                                                 c_node, c_prim, choice);
+                dp_store(dp, p, cur_cost, choice);
+            }
             cur = p;
             p = iparent[p];
         }
@@ -379,8 +485,10 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
 __global__ __launch_bounds__(kBlock) void k_refit_crossing(const int32_t* pending, const uint32_t* pending_count,
                                                            const uint32_t* sorted, const int32_t* idx,
                                                            const float* pos, const int32_t* left,
-                                                           const int32_t* right, const int32_t* iparent,
-                                                           float* ibox, uint32_t* flags)
+                                                           const int32_t* right, const int32_t* first,
+                                                           const int32_t* last, const int32_t* iparent,
+                                                           float* ibox, uint32_t* flags, uint32_t* dp, float c_node,
+                                                           float c_prim)
 {
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= *pending_count) return;
@@ -391,9 +499,18 @@ __global__ __launch_bounds__(kBlock) void k_refit_crossing(const int32_t* pendin
         __threadfence();
         Box a = child_box(left[p], sorted, idx, pos, ibox);
         Box b = child_box(right[p], sorted, idx, pos, ibox);
+        Box u;
         for (int c = 0; c < 3; ++c) {
-            ibox[6 * (int64_t)p + c] = fminf(a.lo[c], b.lo[c]);
-            ibox[6 * (int64_t)p + 3 + c] = fmaxf(a.hi[c], b.hi[c]);
+            u.lo[c] = fminf(a.lo[c], b.lo[c]);
+            u.hi[c] = fmaxf(a.hi[c], b.hi[c]);
+            ibox[6 * (int64_t)p + c] = u.lo[c];
+            ibox[6 * (int64_t)p + 3 + c] = u.hi[c];
+        }
+        if (dp) {
+            uint32_t choice;
+            const DpCost c = dp_combine(child_cost(left[p], a, dp, c_prim), child_cost(right[p], b, dp, c_prim),
+                                        box_half_area(u), last[p] - first[p] + 1, c_node, c_prim, choice);
+            dp_store(dp, p, c, choice);
         }
         p = iparent[p];
     }
@@ -409,17 +526,52 @@ __device__ __forceinline__ float half_area(const float* b)
 __global__ __launch_bounds__(kBlock) void k_collapse_plan(const int32_t* tasks, int ntasks,
                                                           const int32_t* left, const int32_t* right,
                                                           const int32_t* first, const int32_t* last,
-                                                          const float* ibox, int32_t* slots,
+                                                          const float* ibox, const uint32_t* dp, int32_t* slots,
                                                           uint32_t* n_int, uint32_t* n_tri)
 {
     int t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= ntasks) return;
     int32_t b = tasks[t];
     int32_t s[8];
-    s[0] = left[b];
-    s[1] = right[b];
-    int ns = 2;
-    while (ns < 8) {
+    int ns = 0;
+    if (dp) {
+        // follow the cost choices: (node, slots) pairs; the slot counts on the
+        // stack plus the slots taken always sum to at most 8
+        int32_t st_n[8];
+        int st_j[8];
+        const uint32_t k8 = dp[(int64_t)kDpWords * b + 7] & 7u;
+        st_n[0] = right[b];
+        st_j[0] = 8 - (int)k8;
+        st_n[1] = left[b];
+        st_j[1] = (int)k8;
+        int sp = 2;
+        while (sp > 0) {
+            --sp;
+            const int32_t c = st_n[sp];
+            int j = st_j[sp];
+            if (c < 0) {
+                s[ns++] = c;
+                continue;
+            }
+            const uint32_t ch = dp[(int64_t)kDpWords * c + 7];
+            while (j > 1 && ((ch >> (3 * (j - 1))) & 7u) == 0u) --j;
+            if (j == 1) {
+                s[ns++] = c;
+                continue;
+            }
+            const int k = (int)((ch >> (3 * (j - 1))) & 7u);
+            st_n[sp] = right[c];
+            st_j[sp] = j - k;
+            st_n[sp + 1] = left[c];
+            st_j[sp + 1] = k;
+            sp += 2;
+        }
+    } else {
+        s[0] = left[b];
+        s[1] = right[b];
+        ns = 2;
+    }
+    while (!dp && ns < 8) {
         int best = -1;
         float best_a = -1.0f;
         for (int k = 0; k < ns; ++k) {
@@ -639,6 +791,8 @@ int build_bvh8(prt_hip_ctx* ctx, SceneBuffers& sb, prt_build_stats* stats)
     PRT_TRY(ctx, dev_alloc(n_tri, 4 * ni));
     PRT_TRY(ctx, dev_alloc(scan_int, 4 * ni));
     PRT_TRY(ctx, dev_alloc(scan_tri, 4 * ni));
+    DevBuf dp;
+    if (ctx->bvh_collapse == PRT_COLLAPSE_SAH) PRT_TRY(ctx, dev_alloc(dp, 4 * kDpWords * ni));
 
     PRT_TRY(ctx, hipEventRecord(ctx->ev0, s));
 
@@ -713,16 +867,20 @@ int build_bvh8(prt_hip_ctx* ctx, SceneBuffers& sb, prt_build_stats* stats)
         PRT_TRY(ctx, hipMemsetAsync(flags.p, 0, 4 * (ni + 1), s));
         uint32_t* pending_count = flags.as<uint32_t>() + ni;
         const int tiles = blocks_for(n, kRefitTile);
+        uint32_t* dpw = ctx->bvh_collapse == PRT_COLLAPSE_SAH ? dp.as<uint32_t>() : nullptr;
+        const float c_node = 1.0f, c_prim = ctx->bvh_prim_cost;
         k_refit_local<<<tiles, kRefitThreads, 0, s>>>(n, sv, sb.d_idx, sb.d_pos, left.as<int32_t>(),
                                                       right.as<int32_t>(), first.as<int32_t>(), last.as<int32_t>(),
                                                       iparent.as<int32_t>(), lparent.as<int32_t>(), ibox.as<float>(),
-                                                      pending.as<int32_t>(), pending_count);
+                                                      pending.as<int32_t>(), pending_count, dpw, c_node, c_prim);
         PRT_TRY(ctx, hipGetLastError());
         // at most two arrivals per crossing node; bounded by the internal node count
         k_refit_crossing<<<blocks_for(2 * ni, kBlock), kBlock, 0, s>>>(pending.as<int32_t>(), pending_count, sv,
                                                                    sb.d_idx, sb.d_pos, left.as<int32_t>(),
-                                                                   right.as<int32_t>(), iparent.as<int32_t>(),
-                                                                   ibox.as<float>(), flags.as<uint32_t>());
+                                                                   right.as<int32_t>(), first.as<int32_t>(),
+                                                                   last.as<int32_t>(), iparent.as<int32_t>(),
+                                                                   ibox.as<float>(), flags.as<uint32_t>(), dpw,
+                                                                   c_node, c_prim);
         PRT_TRY(ctx, hipGetLastError());
 
         // collapse, one level per iteration
@@ -734,7 +892,7 @@ int build_bvh8(prt_hip_ctx* ctx, SceneBuffers& sb, prt_build_stats* stats)
             int nb = blocks_for(ntasks, kBlock);
             k_collapse_plan<<<nb, kBlock, 0, s>>>(tasks[tc].as<int32_t>(), (int)ntasks, left.as<int32_t>(),
                                                   right.as<int32_t>(), first.as<int32_t>(),
-                                                  last.as<int32_t>(), ibox.as<float>(),
+                                                  last.as<int32_t>(), ibox.as<float>(), dpw,
                                                   slots.as<int32_t>(), n_int.as<uint32_t>(),
                                                   n_tri.as<uint32_t>());
             PRT_TRY(ctx, hipGetLastError());

@@ -200,6 +200,10 @@ struct prt_hip_ctx {
     float* d_albedo_own = nullptr;
     const float* d_albedo = nullptr;
     hipEvent_t kev[4 * 16] = {};    // per-launch timing events (trace/shade x begin/end)
+
+    // BVH8 collapse (prt_hip_set_bvh_collapse)
+    int bvh_collapse = PRT_COLLAPSE_SAH;
+    float bvh_prim_cost = 1.0f;     // triangle test cost relative to a wide-node visit
 };
 
 namespace prt {

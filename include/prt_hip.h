@@ -157,6 +157,17 @@ int prt_hip_scene_attributes(prt_hip_ctx* ctx, const float* normals3, const floa
  * boxes, collapse to a BVH8 with 8-bit quantised child boxes. */
 int prt_hip_bvh_build(prt_hip_ctx* ctx, prt_build_stats* out_stats);
 
+/* Collapse of the binary LBVH into the BVH8 (no counterpart in the reference,
+ * whose build is inside OptiX).  PRT_COLLAPSE_SAH (default): the slots of each
+ * wide node follow the least surface-area cost, computed bottom-up during the
+ * refit (Ylitie, Karras, Laine 2017, section 4.1), with a triangle test
+ * costing prim_cost wide-node visits.  PRT_COLLAPSE_GREEDY: the widest-area
+ * internal child is opened until 8 slots are used.  Applies to later builds;
+ * hits and images do not depend on it. */
+#define PRT_COLLAPSE_GREEDY 0
+#define PRT_COLLAPSE_SAH    1
+int prt_hip_set_bvh_collapse(prt_hip_ctx* ctx, int method, float prim_cost);
+
 /* --- closest hit (IntersectorStreamCuda::intersect) ---------------------- */
 /* Host arrays; copies in, traces, copies out (parity entry). */
 int prt_hip_intersect_closest(prt_hip_ctx* ctx, const prt_ray* rays, prt_hit* hits, int64_t n);

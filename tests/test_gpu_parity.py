@@ -79,9 +79,14 @@ def test_raygen_bit_exact(dev, orc, scenes):
         assert np.array_equal(states, ref_states), f"sampler states differ at {w}x{h}"
 
 
-def test_bvh_invariants(dev, scenes):
+@pytest.mark.parametrize("collapse", ["sah", "greedy"])
+def test_bvh_invariants(dev, scenes, collapse):
     mesh = scenes.scene_heightfield_box(1001)
-    stats = _load(dev, mesh)
+    dev.set_bvh_collapse(collapse)
+    try:
+        stats = _load(dev, mesh)
+    finally:
+        dev.set_bvh_collapse("sah")
     nodes_raw, tris, prims, depth = dev.bvh()
     T = mesh.tri_count
     assert stats["tris"] == T and depth == stats["depth"] >= 1
@@ -126,6 +131,74 @@ def test_bvh_invariants(dev, scenes):
     visit(0)
     assert (seen == 1).all(), "some BVH8 node is unreachable or shared"
     assert (covered == 1).all(), "leaf slots do not partition the triangle records"
+
+
+def _exact_sah_cost(dev, c_prim=1.0):
+    """Surface-area cost of the built BVH8 over exact subtree boxes: every wide
+    node costs its box's half area, every leaf slot its box's half area times
+    its triangle count times c_prim."""
+    nodes_raw, tris, _, _ = dev.bvh()
+    nodes = nodes_raw.view(NODE_DT).ravel()
+    tv = tris.reshape(-1, 3, 3).astype(np.float64)
+    tlo, thi = tv.min(1), tv.max(1)
+
+    def half_area(lo, hi):
+        d = hi - lo
+        return d[0] * d[1] + d[1] * d[2] + d[2] * d[0]
+
+    total = 0.0
+    stack = [(0, None)]
+    boxes = {}
+    order = []
+    while stack:                      # iterative post-order over the wide nodes
+        ni, _ = stack.pop()
+        order.append(ni)
+        nd = nodes[ni]
+        k_int = 0
+        for k in range(8):
+            if (int(nd["imask"]) >> k) & 1:
+                stack.append((int(nd["child_base"]) + k_int, ni))
+                k_int += 1
+    for ni in reversed(order):
+        nd = nodes[ni]
+        lo, hi = np.full(3, np.inf), np.full(3, -np.inf)
+        k_int = 0
+        for k in range(8):
+            if (int(nd["imask"]) >> k) & 1:
+                clo, chi = boxes[int(nd["child_base"]) + k_int]
+                k_int += 1
+            elif nd["meta"][k]:
+                off, cnt = int(nd["meta"][k]) & 31, int(nd["meta"][k]) >> 5
+                r0 = int(nd["tri_base"]) + off
+                clo, chi = tlo[r0:r0 + cnt].min(0), thi[r0:r0 + cnt].max(0)
+                total += half_area(clo, chi) * cnt * c_prim
+            else:
+                continue
+            lo, hi = np.minimum(lo, clo), np.maximum(hi, chi)
+        boxes[ni] = (lo, hi)
+        total += half_area(lo, hi)
+    return total, len(nodes)
+
+
+def test_sah_collapse_is_optimal_over_greedy(dev, scenes):
+    """The cost-optimal collapse minimises the surface-area cost over every
+    way of grouping the same binary tree into 8-wide nodes, so its cost can
+    not exceed the greedy collapse's (1e-5 relative slack for the fp32 sums
+    of the bottom-up pass); both give the same hits."""
+    mesh = scenes.scene_mix(1009, 60_000)
+    rays = _random_rays(mesh, 20_000, 5)
+    costs, hits = {}, {}
+    try:
+        for method in ("greedy", "sah"):
+            dev.set_bvh_collapse(method, 1.0)
+            _load(dev, mesh)
+            costs[method] = _exact_sah_cost(dev, 1.0)
+            hits[method] = dev.intersect(rays)
+    finally:
+        dev.set_bvh_collapse("sah")
+    assert costs["sah"][0] <= costs["greedy"][0] * (1 + 1e-5), costs
+    assert costs["sah"][0] < costs["greedy"][0] * 0.99, costs   # and it is a real improvement here
+    assert np.array_equal(hits["sah"].view(np.uint32), hits["greedy"].view(np.uint32))
 
 
 @pytest.mark.parametrize("kind", ["primary", "random"])
