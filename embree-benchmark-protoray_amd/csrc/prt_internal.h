@@ -133,7 +133,7 @@ extern const BatchKernelConfig kBatchVariants[kBatchVariantCount];
 struct BatchTuning {
     int variant = 0;
     int blocks_per_cu = 0;   // 0: the occupancy maximum
-    int refill = 24;
+    int refill = 32;
 };
 
 }  // namespace prt

@@ -41,7 +41,7 @@ namespace prt {
 namespace {
 
 constexpr int kStackLds = 16;      // default LDS entries per lane (power of two)
-constexpr int kRefill = 24;        // default: refill when this many lanes are idle
+constexpr int kRefill = 32;        // default: refill when this many lanes are idle
 
 enum {
     kModeTrace = 0, kModeDiffuse = 1, kModePrimary = 2, kModeBatch = 3, kModeRgb = 4, kModeAdjoint = 5,
