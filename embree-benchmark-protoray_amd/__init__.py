@@ -138,6 +138,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "prt_hip_set_profiling": ([P, ctypes.c_int], ctypes.c_int),
         "prt_hip_set_pipeline": ([P, ctypes.c_int], ctypes.c_int),
         "prt_hip_set_bvh_collapse": ([P, ctypes.c_int, ctypes.c_float], ctypes.c_int),
+        "prt_hip_set_tail": ([P, ctypes.c_int], ctypes.c_int),
         "prt_hip_accum_read": ([P, P], ctypes.c_int),
         "prt_hip_frame_read_bgr8": ([P, P], ctypes.c_int),
         "prt_hip_debug_raygen": ([P, ctypes.c_int, P, P], ctypes.c_int),
@@ -357,6 +358,14 @@ class HipDevice:
                 "nodes": st.nodes, "tris": st.tris, "spills": st.spills, "waveSteps": st.wave_steps,
                 "laneSteps": st.lane_steps, "waveShades": st.wave_shades,
                 "drainWaveSteps": st.drain_wave_steps, "drainLaneSteps": st.drain_lane_steps, "drainMs": st.drain_ms}
+
+    def set_tail(self, mode: str = "cooperative") -> None:
+        """Tail of the fused diffuse frame: 'cooperative' (a wave with at most 8
+        paths left spreads each over 8 lanes) or 'lanes'."""
+        ids = {"lanes": 0, "cooperative": 1}
+        if mode not in ids:
+            raise ValueError("mode must be 'cooperative' or 'lanes'")
+        self._check(self.lib.prt_hip_set_tail(self.ctx, ids[mode]), "set_tail")
 
     def set_pipeline(self, pipeline: str) -> None:
         """'fused' (one persistent path launch per frame) or 'wavefront' (per-depth launches)."""

@@ -486,6 +486,14 @@ int prt_hip_set_profiling(prt_hip_ctx* ctx, int flags)
     return 0;
 }
 
+int prt_hip_set_tail(prt_hip_ctx* ctx, int mode)
+{
+    if (!ctx) return 1;
+    if (mode != PRT_TAIL_LANES && mode != PRT_TAIL_COOPERATIVE) return set_error(ctx, "set_tail: unknown mode");
+    ctx->tail_coop = mode == PRT_TAIL_COOPERATIVE ? 1 : 0;
+    return 0;
+}
+
 int prt_hip_set_pipeline(prt_hip_ctx* ctx, int pipeline)
 {
     if (!ctx) return 1;
@@ -537,6 +545,7 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
         p.depth_rays = ctx->d_stats + 16;
         p.stats = ctx->d_stats;
         p.fast = fast;
+        p.coop = ctx->tail_coop;
         if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[0], ctx->stream));
         PRT_TRY(ctx, launch_paths(ctx, p, renderer, counting));
         if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[1], ctx->stream));

@@ -201,6 +201,9 @@ struct prt_hip_ctx {
     const float* d_albedo = nullptr;
     hipEvent_t kev[4 * 16] = {};    // per-launch timing events (trace/shade x begin/end)
 
+    // cooperative tail of the fused diffuse frame (prt_hip_set_tail)
+    int tail_coop = 1;
+
     // BVH8 collapse (prt_hip_set_bvh_collapse)
     int bvh_collapse = PRT_COLLAPSE_SAH;
     float bvh_prim_cost = 1.0f;     // triangle test cost relative to a wide-node visit
@@ -244,6 +247,9 @@ struct PathLaunch {
     // the "Fast" renderers: frame of the geometric normal whatever attributes
     // the mesh has, and each pass overwrites the accumulator
     bool fast;
+    // diffuse frames: a wave with at most 8 paths left traces them
+    // cooperatively (0: never)
+    int coop;
 };
 int ensure_trace_resources(prt_hip_ctx* ctx);
 hipError_t launch_trace(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumented);

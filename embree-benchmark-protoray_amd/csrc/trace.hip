@@ -133,6 +133,12 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     unsigned long long n_dwsteps = 0, n_dlsteps = 0;              // ... after the work ran out
     unsigned long long t_exhausted = 0;                           // wall clock when it ran out
     bool exhausted = false;  // wave-uniform: the work counter has run past count
+    // Tail of a diffuse frame (kModeDiffuse): once out of work with at most
+    // 8 paths left, the wave traces them cooperatively, 8 lanes per path
+    // (see the cooperative phase after the main loop)
+    constexpr bool kCoop = kMode == kModeDiffuse;
+    const bool coop_on = kCoop && pl.coop != 0;
+    bool cont = false;  // the path's next ray waits in (o, d) for the cooperative phase
 
     auto start_ray = [&](f3 org, f3 dir, float near, float far) {
         o = org;
@@ -181,6 +187,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     };
 
     for (;;) {
+        // the wave goes cooperative once out of work, with at most 8 paths
+        // left and all of them between rays
+        bool go_coop = false;
+        if (coop_on && exhausted) {
+            const uint64_t a = __ballot(active);
+            go_coop = a != 0 && __popcll(a) <= 8 && __ballot(active && tracing) == 0;
+        }
         // ---- shade the lanes whose ray has finished.  The traversal loop
         // hands them over in groups, so this code runs for many lanes at once
         // instead of stalling every traversal step of the wave for one lane.
@@ -329,8 +342,15 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             f3 no, nd;
             diffuse_bounce(o, d, t, fr, s1, s2, no, nd);
             ++depth;
+            if (kCoop && go_coop) {
+                o = no;
+                d = nd;
+                cont = true;
+                break;
+            }
             start_ray(no, nd, 0.0f, 3.402823466e38f);
         } while (0);
+        if (kCoop && go_coop) break;
 
         // ---- refill idle lanes (wave-uniform branch)
         const uint64_t idle = __ballot(!active);
@@ -460,7 +480,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
         // refill lanes once no more than 64 - refill trace; after the work
         // counter ran out, shade once half of the remaining lanes wait
         const int n_act = __popcll(busy);
-        const int keep = exhausted ? n_act - max(1, min(refill, n_act >> 1)) : 64 - refill;
+        const int keep = !exhausted ? 64 - refill
+                                    : (coop_on && n_act <= 8 ? 0 : n_act - max(1, min(refill, n_act >> 1)));
         while (tracing) {
             if (kCount) {
                 const bool lead = lane == __ffsll((unsigned long long)__ballot(1)) - 1;
@@ -651,6 +672,278 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             }
 
             if (__popcll(__ballot(1)) <= keep) break;
+        }
+    }
+
+    // ---- cooperative tail (kModeDiffuse).  Each of the wave's remaining
+    // paths (at most 8) moves to a group of G = 8 lanes that hold its ray
+    // state alike.  Per step, lane j of a group tests child j of the visited
+    // node, or one of G pending triangles, and the group's choices (nearest children, closest hit) come
+    // from shuffles inside the group.  The closest hit is the same (t, primId)
+    // minimum as lane by lane, whatever the order of the tests, so images do
+    // not change; a path's tail of dependent steps gets shorter.  The group's
+    // stack is its lanes' LDS rings joined (G x kStack entries), spilling to
+    // its first lane's slice.
+    if constexpr (kCoop) {
+        const uint64_t cm = __ballot(cont);
+        if (coop_on && cm != 0) {
+            const int npaths = __popcll(cm);
+            constexpr int lg = 3;                 // log2 G: 8 lanes per path
+            constexpr int G = 1 << lg;
+            const int gj = lane & (G - 1);        // this lane's slot in its group
+            const int gbase = lane & ~(G - 1);    // first lane of the group
+            const int grp = lane >> lg;
+            constexpr uint32_t gmask = (1u << G) - 1u;
+            constexpr int kGStack = G * kStack;
+            uint64_t mm = cm;
+            for (int k = 0; k < grp; ++k) mm &= mm - 1;
+            const int src = mm ? __ffsll((unsigned long long)mm) - 1 : 0;
+            const bool gact = grp < npaths;
+            o = mk3(__shfl(o.x, src, 64), __shfl(o.y, src, 64), __shfl(o.z, src, 64));
+            d = mk3(__shfl(d.x, src, 64), __shfl(d.y, src, 64), __shfl(d.z, src, 64));
+            pixel = __shfl(pixel, src, 64);
+            rng = __shfl(rng, src, 64);
+            depth = __shfl(depth, src, 64);
+            lw = __shfl(lw, src, 64);
+            cont = false;
+            active = gact;
+            tracing = false;
+            if (gact) start_ray(o, d, 0.0f, 3.402823466e38f);
+            uint2* g_spill = spill + ((size_t)blockIdx.x * kBlock + (tid & ~(G - 1))) * (size_t)stack_entries;
+            auto gslot = [&](int q) { return ((q >> lg) & (kStack - 1)) * kBlock + (tid & ~(G - 1)) + (q & (G - 1)); };
+            for (;;) {
+                while (__ballot(tracing) != 0) {
+                    if (kCount) {
+                        const bool lead = lane == __ffsll((unsigned long long)__ballot(1)) - 1;
+                        if (tracing) {
+                            ++n_lsteps;
+                            ++n_dlsteps;
+                        }
+                        if (lead) {
+                            ++n_wsteps;
+                            ++n_dwsteps;
+                        }
+                    }
+                    if (tracing && pend == 0) {
+                        if (node >= 0 && node_t > tbest) node = -1;
+                        if (node < 0) {
+                            while (sp > 0) {
+                                --sp;
+                                uint32_t nid;
+                                float nt;
+                                if (sp < bottom) {
+                                    const uint2 e = g_spill[sp];
+                                    bottom = sp;
+                                    nid = e.x;
+                                    nt = as_f(e.y);
+                                } else {
+                                    const int slot = gslot(sp & (kGStack - 1));
+                                    nid = st_node[slot];
+                                    nt = st_t[slot];
+                                }
+                                if (nt <= tbest) { node = (int32_t)nid; break; }
+                            }
+                            if (node < 0) tracing = false;
+                        }
+                    }
+                    const bool node_step = tracing && pend == 0;
+                    // this lane's children: slots gj + c G, c < 8 / G
+                    float tkc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                    bool hitc[4] = {false, false, false, false};
+                    uint4 w0{}, w1{};
+                    if (node_step) {
+                        const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
+                        w0 = np[0];
+                        w1 = np[1];
+                        const uint4 w2 = np[2], w3 = np[3], w4 = np[4];
+                        if (kCount && gj == 0) ++n_nodes;
+                        const int ex = (int)(int8_t)(w0.w & 0xffu);
+                        const int ey = (int)(int8_t)((w0.w >> 8) & 0xffu);
+                        const int ez = (int)(int8_t)((w0.w >> 16) & 0xffu);
+                        const float ax = __uint_as_float((uint32_t)(ex + 127) << 23) * inv_x;
+                        const float ay = __uint_as_float((uint32_t)(ey + 127) << 23) * inv_y;
+                        const float az = __uint_as_float((uint32_t)(ez + 127) << 23) * inv_z;
+                        const float bx = (as_f(w0.x) - o.x) * inv_x;
+                        const float by = (as_f(w0.y) - o.y) * inv_y;
+                        const float bz = (as_f(w0.z) - o.z) * inv_z;
+                        const float slx = 4.76837158203125e-7f * (255.0f * fabsf(ax) + fabsf(bx));
+                        const float sly = 4.76837158203125e-7f * (255.0f * fabsf(ay) + fabsf(by));
+                        const float slz = 4.76837158203125e-7f * (255.0f * fabsf(az) + fabsf(bz));
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            const int k = gj + (c << lg);
+                            if (k < 8) {
+                                // child k: byte k & 3 of the words of its half of the node
+                                const bool hi = k >= 4;
+                                const int kb = k & 3;
+                                const uint32_t qlx = hi ? w2.y : w2.x, qhx = hi ? w3.w : w3.z;
+                                const uint32_t qly = hi ? w2.w : w2.z, qhy = hi ? w4.y : w4.x;
+                                const uint32_t qlz = hi ? w3.y : w3.x, qhz = hi ? w4.w : w4.z;
+                                const float lox = byte_f(qlx, kb), hix = byte_f(qhx, kb);
+                                const float loy = byte_f(qly, kb), hiy = byte_f(qhy, kb);
+                                const float loz = byte_f(qlz, kb), hiz = byte_f(qhz, kb);
+                                const float tnx = __builtin_fmaf(neg_x ? hix : lox, ax, bx - slx);
+                                const float tny = __builtin_fmaf(neg_y ? hiy : loy, ay, by - sly);
+                                const float tnz = __builtin_fmaf(neg_z ? hiz : loz, az, bz - slz);
+                                const float tfx = __builtin_fmaf(neg_x ? lox : hix, ax, bx + slx);
+                                const float tfy = __builtin_fmaf(neg_y ? loy : hiy, ay, by + sly);
+                                const float tfz = __builtin_fmaf(neg_z ? loz : hiz, az, bz + slz);
+                                tkc[c] = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
+                                const float tf = fminf(fminf(fminf(tfx, tfy), tfz), tbest);
+                                hitc[c] = tkc[c] <= tf;
+                            }
+                        }
+                    }
+                    uint32_t ghits = 0;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        ghits |= ((uint32_t)(__ballot(hitc[c]) >> gbase) & gmask) << (c << lg);
+                    ghits &= 0xffu;
+                    if (node_step) {
+                        const uint32_t imask = w0.w >> 24;
+                        const uint32_t hit_int = ghits & imask;
+                        uint32_t hit_leaf = ghits & ~imask;
+                        while (hit_leaf) {
+                            const int k = __builtin_ctz(hit_leaf);
+                            hit_leaf &= hit_leaf - 1;
+                            const uint32_t mb = ((k < 4 ? w1.z : w1.w) >> (8 * (k & 3))) & 0xffu;
+                            pend |= ((1u << (mb >> 5)) - 1u) << (mb & 31u);
+                        }
+                        pend_base = w1.y;
+                        node = -1;
+                        if (hit_int) {
+                            // nearest and second nearest internal child over the group
+                            float t1 = __builtin_inff(), t2 = __builtin_inff();
+                            int k1 = 8, k2 = 8;
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                const int k = gj + (c << lg);
+                                if (k < 8 && ((hit_int >> k) & 1u) && (tkc[c] < t1 || (tkc[c] == t1 && k < k1))) {
+                                    t1 = tkc[c];
+                                    k1 = k;
+                                }
+                            }
+                            for (int x = 1; x < G; x <<= 1) {
+                                const float ot = __shfl_xor(t1, x, 64);
+                                const int ok = __shfl_xor(k1, x, 64);
+                                if (ot < t1 || (ot == t1 && ok < k1)) { t1 = ot; k1 = ok; }
+                            }
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                const int k = gj + (c << lg);
+                                if (k < 8 && k != k1 && ((hit_int >> k) & 1u) &&
+                                    (tkc[c] < t2 || (tkc[c] == t2 && k < k2))) {
+                                    t2 = tkc[c];
+                                    k2 = k;
+                                }
+                            }
+                            for (int x = 1; x < G; x <<= 1) {
+                                const float ot = __shfl_xor(t2, x, 64);
+                                const int ok = __shfl_xor(k2, x, 64);
+                                if (ot < t2 || (ot == t2 && ok < k2)) { t2 = ot; k2 = ok; }
+                            }
+                            const uint32_t child_base = w1.x;
+                            node = (int32_t)(child_base + __popc(imask & ((1u << k1) - 1u)));
+                            node_t = t1;
+                            const uint32_t pushm = hit_int & ~(1u << k1);
+                            if (pushm) {
+                                const int npush = __popc(pushm);
+                                bool ok = true;
+                                while (sp + npush - bottom > kGStack) {
+                                    if (bottom >= spill_cap) { atomicOr(err, 1u); ok = false; break; }
+                                    const int bslot = gslot(bottom & (kGStack - 1));
+                                    g_spill[bottom] = make_uint2(st_node[bslot], __float_as_uint(st_t[bslot]));
+                                    ++bottom;
+                                    if (kCount && gj == 0) ++n_spill;
+                                }
+                                if (ok) {
+                                    const int klast = 31 - __builtin_clz(pushm);
+                                    if (k2 > 7 || !((pushm >> k2) & 1u)) k2 = klast;
+                                    const int pos_k2 = __popc(pushm & ((1u << k2) - 1u));
+#pragma unroll
+                                    for (int c = 0; c < 4; ++c) {
+                                        const int k = gj + (c << lg);
+                                        if (k < 8 && ((pushm >> k) & 1u)) {
+                                            int pos = __popc(pushm & ((1u << k) - 1u));
+                                            pos = k == k2 ? npush - 1 : (k == klast ? pos_k2 : pos);
+                                            const int slot = gslot((sp + pos) & (kGStack - 1));
+                                            st_node[slot] = child_base + __popc(imask & ((1u << k) - 1u));
+                                            st_t[slot] = tkc[c];
+                                        }
+                                    }
+                                    sp += npush;
+                                }
+                            }
+                        }
+                    }
+                    if (tracing && pend != 0 && !node_step) {
+                        // up to G pending triangles at once: lane j takes the
+                        // j-th lowest pending record
+                        uint32_t mj = pend;
+                        for (int k = 0; k < gj; ++k) mj &= mj - 1;
+                        for (int k = 0; k < G; ++k) pend &= pend - 1;
+                        float ct = __builtin_inff();
+                        int32_t cp = INT32_MAX;
+                        uint32_t crec = 0;
+                        if (mj != 0) {
+                            const uint32_t rec = pend_base + (uint32_t)__builtin_ctz(mj);
+                            const float* tp = tris + 9 * (size_t)rec;
+                            const f3 p0 = mk3(tp[0], tp[1], tp[2]);
+                            const f3 p1 = mk3(tp[3], tp[4], tp[5]);
+                            const f3 p2 = mk3(tp[6], tp[7], tp[8]);
+                            if (kCount) ++n_tris;
+                            TriCandidate c;
+                            if (tri_test(wr, o, d, p0, p1, p2, tmin, tbest, c)) {
+                                ct = c.t;
+                                cp = prim_ids[rec];
+                                crec = rec;
+                            }
+                        }
+                        for (int x = 1; x < G; x <<= 1) {
+                            const float ot = __shfl_xor(ct, x, 64);
+                            const int32_t op = __shfl_xor(cp, x, 64);
+                            const uint32_t orr = __shfl_xor(crec, x, 64);
+                            if (ot < ct || (ot == ct && op < cp)) { ct = ot; cp = op; crec = orr; }
+                        }
+                        if (cp != INT32_MAX && (ct < tbest || best_prim < 0 || cp < best_prim)) {
+                            tbest = ct;
+                            best_prim = cp;
+                            best_rec = crec;
+                        }
+                    }
+                }
+                // ---- shade each group's finished ray (the same in its G lanes;
+                // its first lane counts the ray and writes the pixel)
+                if (active) {
+                    if constexpr (kCount) {
+                        if (gj == 0) atomicAdd(&depth_hist[depth], 1u);
+                    }
+                    if (gj == 0) ++n_rays;
+                    float4* acc = pl.accum + pixel;
+                    if (best_prim < 0 || depth == max_depth) {
+                        const float c = best_prim < 0 ? lw : 0.0f;
+                        if (gj == 0) {
+                            const float4 a = pl.fast ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : *acc;
+                            *acc = make_float4(a.x + c, a.y + c, a.z + c, a.w + 1.0f);
+                        }
+                        active = false;
+                    } else {
+                        const float* tp = tris + 9 * (size_t)best_rec;
+                        const f3 p0 = mk3(tp[0], tp[1], tp[2]);
+                        const f3 p1 = mk3(tp[3], tp[4], tp[5]);
+                        const f3 p2 = mk3(tp[6], tp[7], tp[8]);
+                        lw *= albedo;
+                        const Frame fr = hit_frame(p0, p1, p2);
+                        const float s1 = sampler_next(rng);
+                        const float s2 = sampler_next(rng);
+                        f3 no, nd;
+                        diffuse_bounce(o, d, tbest, fr, s1, s2, no, nd);
+                        ++depth;
+                        start_ray(no, nd, 0.0f, 3.402823466e38f);
+                    }
+                }
+                if (__ballot(active) == 0) break;
+            }
         }
     }
 

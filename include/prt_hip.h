@@ -229,6 +229,14 @@ int prt_hip_set_profiling(prt_hip_ctx* ctx, int flags);
 #define PRT_PIPELINE_FUSED     0
 #define PRT_PIPELINE_WAVEFRONT 1
 int prt_hip_set_pipeline(prt_hip_ctx* ctx, int pipeline);
+/* Tail of the fused diffuse frame (no counterpart in the reference): once the
+ * frame's paths are all handed out and a wave holds at most 8 of them,
+ * PRT_TAIL_COOPERATIVE (default) traces each on 8 lanes (one BVH8 child, or
+ * one of up to 8 pending triangles, per lane and step); PRT_TAIL_LANES keeps
+ * one lane per path.  Images do not change. */
+#define PRT_TAIL_LANES       0
+#define PRT_TAIL_COOPERATIVE 1
+int prt_hip_set_tail(prt_hip_ctx* ctx, int mode);
 
 /* --- frame buffer read-back -------------------------------------------- */
 /* Raw accumulation buffer, float4 (r, g, b, weight) per pixel, row-major. */
