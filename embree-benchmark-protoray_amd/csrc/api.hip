@@ -200,11 +200,13 @@ int prt_hip_create(int device_index, prt_hip_ctx** out_ctx)
         hipGetDeviceProperties(&prop, device_index) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
-        hipMalloc(&ctx->d_counters, 64 * sizeof(uint32_t)) != hipSuccess ||  // [48] batch work counter
-        hipMalloc(&ctx->d_stats, 32 * sizeof(unsigned long long)) != hipSuccess) {
+        // counters ([48] batch work counter) and statistics share one
+        // allocation, so a frame reads both back with one copy
+        hipMalloc(&ctx->d_counters, 64 * sizeof(uint32_t) + 32 * sizeof(unsigned long long)) != hipSuccess) {
         prt_hip_destroy(ctx);
         return 5;
     }
+    ctx->d_stats = reinterpret_cast<unsigned long long*>(ctx->d_counters + 64);
     ctx->num_cus = prop.multiProcessorCount;
     ctx->device_name = prop.name[0] ? std::string(prop.name) : std::string("AMD ") + prop.gcnArchName;
     if (hipMemsetAsync(ctx->d_counters, 0, 64 * sizeof(uint32_t), ctx->stream) != hipSuccess ||
@@ -227,7 +229,7 @@ int prt_hip_destroy(prt_hip_ctx* ctx)
     void* ptrs[] = {ctx->d_spill,
                     ctx->d_accum, ctx->d_bgr8, ctx->d_rays[0], ctx->d_rays[1], ctx->d_hits,
                     ctx->d_pix[0], ctx->d_pix[1], ctx->d_rng[0], ctx->d_rng[1], ctx->d_counters,
-                    ctx->d_stats, ctx->d_io_rays, ctx->d_io_hits};
+                    ctx->d_io_rays, ctx->d_io_hits};
     for (void* p : ptrs) free_ptr(p);
     for (hipEvent_t e : ctx->kev)
         if (e) (void)hipEventDestroy(e);
@@ -573,10 +575,14 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
         }
     }
     PRT_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    uint32_t counters[33];
-    unsigned long long st[32];
-    PRT_TRY(ctx, hipMemcpyAsync(counters, ctx->d_counters, sizeof(counters), hipMemcpyDeviceToHost, ctx->stream));
-    PRT_TRY(ctx, hipMemcpyAsync(st, ctx->d_stats, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+    struct {
+        uint32_t counters[64];
+        unsigned long long st[32];
+    } rb;
+    static_assert(sizeof(rb) == 64 * sizeof(uint32_t) + 32 * sizeof(unsigned long long), "read-back layout");
+    PRT_TRY(ctx, hipMemcpyAsync(&rb, ctx->d_counters, sizeof(rb), hipMemcpyDeviceToHost, ctx->stream));
+    const uint32_t* counters = rb.counters;
+    const unsigned long long* st = rb.st;
     PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
     if (counters[32]) return set_error(ctx, "render: traversal stack overflow");
     float ms = 0.0f;
