@@ -671,7 +671,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                 }
             }
 
-            if (__popcll(__ballot(1)) <= keep) break;
+            if ((int)__popcll(__ballot(1)) <= keep) break;
         }
     }
 
