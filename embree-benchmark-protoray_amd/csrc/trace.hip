@@ -100,6 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     int32_t node = -1;
     float node_t = 0.0f;        // entry distance of `node`
     uint32_t pend = 0, pend_base = 0;  // triangle records still to test: pend_base + bit index
+    uint32_t pend2 = 0, pend2_base = 0;  // a second leaf group, found while the first is tested
     // path state (fused modes)
     int32_t pixel = 0;
     uint32_t rng = 0;
@@ -161,6 +162,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
         node = 0;
         node_t = near;
         pend = 0;
+        pend2 = 0;
         tracing = true;
     };
 
@@ -492,7 +494,48 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                     if (lead) ++n_dwsteps;
                 }
             }
-            if (pend == 0) {
+            // A step tests one pending triangle and visits one node: a lane
+            // keeps descending while its leaf triangles are tested, with room
+            // for one more leaf group.  The test comes first, so the node's
+            // culling sees its result.
+            if (pend != 0) {
+                // ---- one triangle of the pending group per step, so lanes
+                // with many candidate triangles do not hold up the wave
+                const uint32_t rec = pend_base + (uint32_t)__builtin_ctz(pend);
+                pend &= pend - 1;
+                const float* tp = tris + 9 * (size_t)rec;
+                const f3 p0 = mk3(tp[0], tp[1], tp[2]);
+                const f3 p1 = mk3(tp[3], tp[4], tp[5]);
+                const f3 p2 = mk3(tp[6], tp[7], tp[8]);
+                if (kCount) ++n_tris;
+                TriCandidate c;
+                if (tri_test(wr, o, d, p0, p1, p2, tmin, tbest, c)) {
+                    if (kMode == kModeOccluded || (kMode == kModeDiffuse2 && shadow)) {
+                        // any hit ends the query: drop the stack and the pending work
+                        tbest = c.t;
+                        best_prim = 0;
+                        sp = bottom = 0;
+                        pend = pend2 = 0;
+                        node = -1;
+                    } else {
+                        const int32_t prim = prim_ids[rec];
+                        if (c.t < tbest || best_prim < 0 || prim < best_prim) {
+                            tbest = c.t;
+                            best_prim = prim;
+                            best_rec = rec;
+                            bu = c.v_num / c.det;
+                            bv = c.w_num / c.det;
+                        }
+                    }
+                }
+                if (pend == 0 && pend2 != 0) {
+                    pend = pend2;
+                    pend_base = pend2_base;
+                    pend2 = 0;
+                }
+            }
+
+            if (pend2 == 0) {
                 // ---- node step: the next node that can still hold a closer hit
                 if (node >= 0 && node_t > tbest) node = -1;
                 if (node < 0) {
@@ -512,11 +555,14 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                         }
                         if (nt <= tbest) { node = (int32_t)nid; break; }
                     }
-                    if (node < 0) {
-                        tracing = false;  // the ray is finished: shade it in the outer loop
-                        break;
-                    }
                 }
+            }
+            if (pend2 == 0 && node < 0) {
+                if (pend == 0) {
+                    tracing = false;  // the ray is finished: shade it in the outer loop
+                    break;
+                }
+            } else if (pend2 == 0) {
 
                 // ---- visit one BVH8 node
                 const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
@@ -573,14 +619,24 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                 uint32_t hit_leaf = hits & ~imask;
 
                 // ---- leaf slots: one pending mask over the node's triangle
-                // records (a node owns at most 8 x kLeafMax <= 32 of them)
+                // records (a node owns at most 8 x kLeafMax <= 32 of them),
+                // the first group if it is free, else the second
+                uint32_t newp = 0;
                 while (hit_leaf) {
                     const int k = __builtin_ctz(hit_leaf);
                     hit_leaf &= hit_leaf - 1;
                     const uint32_t mb = ((k < 4 ? meta0 : meta1) >> (8 * (k & 3))) & 0xffu;
-                    pend |= ((1u << (mb >> 5)) - 1u) << (mb & 31u);
+                    newp |= ((1u << (mb >> 5)) - 1u) << (mb & 31u);
                 }
-                pend_base = w1.y;
+                if (newp != 0) {
+                    if (pend == 0) {
+                        pend = newp;
+                        pend_base = w1.y;
+                    } else {
+                        pend2 = newp;
+                        pend2_base = w1.y;
+                    }
+                }
 
                 // ---- internal slots: the nearest is visited next, the others
                 // are pushed with the second nearest on top.  Unrolled over the
@@ -634,38 +690,6 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                                 }
                             }
                             sp += npush;
-                        }
-                    }
-                }
-            }
-
-            if (pend != 0) {
-                // ---- one triangle of the pending group per step, so lanes
-                // with many candidate triangles do not hold up the wave
-                const uint32_t rec = pend_base + (uint32_t)__builtin_ctz(pend);
-                pend &= pend - 1;
-                const float* tp = tris + 9 * (size_t)rec;
-                const f3 p0 = mk3(tp[0], tp[1], tp[2]);
-                const f3 p1 = mk3(tp[3], tp[4], tp[5]);
-                const f3 p2 = mk3(tp[6], tp[7], tp[8]);
-                if (kCount) ++n_tris;
-                TriCandidate c;
-                if (tri_test(wr, o, d, p0, p1, p2, tmin, tbest, c)) {
-                    if (kMode == kModeOccluded || (kMode == kModeDiffuse2 && shadow)) {
-                        // any hit ends the query: drop the stack and the pending work
-                        tbest = c.t;
-                        best_prim = 0;
-                        sp = bottom = 0;
-                        pend = 0;
-                        node = -1;
-                    } else {
-                        const int32_t prim = prim_ids[rec];
-                        if (c.t < tbest || best_prim < 0 || prim < best_prim) {
-                            tbest = c.t;
-                            best_prim = prim;
-                            best_rec = rec;
-                            bu = c.v_num / c.det;
-                            bv = c.w_num / c.det;
                         }
                     }
                 }
