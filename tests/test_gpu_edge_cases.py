@@ -135,6 +135,14 @@ def test_error_paths_leave_the_context_usable(prt, orc, scenes):
             d.lib.prt_hip_render(d.ctx, 7, 4, 0.8, None) == 0 or d._check(1, "render")
         with pytest.raises(ValueError):
             d.render("noSuchRenderer")
+        with pytest.raises(RuntimeError, match="set_bvh_collapse"):
+            d.lib.prt_hip_set_bvh_collapse(d.ctx, 7, 1.0) == 0 or d._check(1, "set_bvh_collapse")
+        with pytest.raises(RuntimeError, match="set_bvh_collapse"):
+            d.lib.prt_hip_set_bvh_collapse(d.ctx, 1, -1.0) == 0 or d._check(1, "set_bvh_collapse")
+        with pytest.raises(RuntimeError, match="set_tail"):
+            d.lib.prt_hip_set_tail(d.ctx, 9) == 0 or d._check(1, "set_tail")
+        with pytest.raises(ValueError):
+            d.set_tail("sideways")
         # still usable after every failure
         d.render("diffuseHip", max_depth=4, albedo=0.8)
         ref = orc.Scene(mesh.idx, mesh.pos).render(cam, 16, 16, spp=1, max_depth=4, albedo=0.8, renderer=1,
