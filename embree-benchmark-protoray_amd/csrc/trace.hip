@@ -644,24 +644,27 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                 // whatever the number of hit children per lane.
                 node = -1;
                 if (hit_int) {
-                    // a hit child has a finite entry distance (tn <= tbest), so
-                    // k1 and, with two or more hits, k2 are always hit slots
-                    float t1 = __builtin_inff(), t2 = __builtin_inff();
-                    int k1 = 0, k2 = 0;
+                    // nearest and second-nearest hit internal child by integer
+                    // minimum over keys (entry distance bits with the slot in
+                    // the low 3 bits; the distances are >= 0, and a -0 loses
+                    // its sign bit).  Ties and distances within 8 ulps order by
+                    // slot; node_t is the key's distance, at most 7 ulps below
+                    // the true one, so culling with it stays conservative.
+                    uint32_t key[8];
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        const bool h = (hit_int >> k) & 1u;
-                        const float t = tk[k];
-                        const bool lt1 = h && t < t1;
-                        const bool lt2 = h && !lt1 && t < t2;
-                        t2 = lt1 ? t1 : (lt2 ? t : t2);
-                        k2 = lt1 ? k1 : (lt2 ? k : k2);
-                        t1 = lt1 ? t : t1;
-                        k1 = lt1 ? k : k1;
-                    }
+                    for (int k = 0; k < 8; ++k)
+                        key[k] = ((hit_int >> k) & 1u) ? ((__float_as_uint(tk[k]) & 0x7ffffff8u) | (uint32_t)k)
+                                                      : 0xffffffffu;
+                    const uint32_t m1 = min(min(min(key[0], key[1]), min(key[2], key[3])),
+                                            min(min(key[4], key[5]), min(key[6], key[7])));
+                    uint32_t m2 = 0xffffffffu;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) m2 = min(m2, key[k] == m1 ? 0xffffffffu : key[k]);
+                    const int k1 = (int)(m1 & 7u);
+                    int k2 = (int)(m2 & 7u);
                     const uint32_t child_base = w1.x;
                     node = (int32_t)(child_base + __popc(imask & ((1u << k1) - 1u)));
-                    node_t = t1;
+                    node_t = __uint_as_float(m1 & ~7u);
                     const uint32_t pushm = hit_int & ~(1u << k1);
                     if (pushm) {
                         const int npush = __popc(pushm);
