@@ -657,11 +657,16 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                                                       : 0xffffffffu;
                     const uint32_t m1 = min(min(min(key[0], key[1]), min(key[2], key[3])),
                                             min(min(key[4], key[5]), min(key[6], key[7])));
-                    uint32_t m2 = 0xffffffffu;
+                    // second smallest: no other key equals m1, so key - m1 - 1
+                    // (mod 2^32) maps m1 to the largest value and keeps the
+                    // order of the keys above it
+                    uint32_t e[8];
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) m2 = min(m2, key[k] == m1 ? 0xffffffffu : key[k]);
+                    for (int k = 0; k < 8; ++k) e[k] = key[k] + ~m1;
+                    const uint32_t m2 = min(min(min(e[0], e[1]), min(e[2], e[3])), min(min(e[4], e[5]), min(e[6], e[7]))) +
+                                        m1 + 1u;
                     const int k1 = (int)(m1 & 7u);
-                    int k2 = (int)(m2 & 7u);
+                    const int k2 = (int)(m2 & 7u);
                     const uint32_t child_base = w1.x;
                     node = (int32_t)(child_base + __popc(imask & ((1u << k1) - 1u)));
                     node_t = __uint_as_float(m1 & ~7u);
@@ -679,19 +684,26 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                             if (kCount) ++n_spill;
                         }
                         if (ok) {
-                            const int klast = 31 - __builtin_clz(pushm);
-                            if (!((pushm >> k2) & 1u)) k2 = klast;
-                            const int pos_k2 = __popc(pushm & ((1u << k2) - 1u));
+                            // branch-free: slot k goes to sp + (pushed slots
+                            // below k, k2 left out).  A slot that is not pushed
+                            // (or is k2) lands on the position of the next
+                            // pushed slot above it, written later, or on the
+                            // top, written last with k2's entry; so every
+                            // write stays inside [sp, sp + npush).  k2 is in
+                            // pushm: it is the second hit internal child.
+                            const uint32_t pm2 = pushm & ~(1u << k2);
 #pragma unroll
                             for (int k = 0; k < 8; ++k) {
-                                if ((pushm >> k) & 1u) {
-                                    int pos = __popc(pushm & ((1u << k) - 1u));
-                                    pos = k == k2 ? npush - 1 : (k == klast ? pos_k2 : pos);
-                                    const int slot = ((sp + pos) & (kStack - 1)) * kBlock + tid;
-                                    st_node[slot] = child_base + __popc(imask & ((1u << k) - 1u));
-                                    st_t[slot] = tk[k];
-                                }
+                                const int pos = __popc(pm2 & ((1u << k) - 1u));
+                                const int slot = ((sp + pos) & (kStack - 1)) * kBlock + tid;
+                                st_node[slot] = child_base + __popc(imask & ((1u << k) - 1u));
+                                st_t[slot] = tk[k];
                             }
+                            // k2 on top, with its key's distance (at most 7
+                            // ulps below its entry distance: conservative)
+                            const int slot = ((sp + npush - 1) & (kStack - 1)) * kBlock + tid;
+                            st_node[slot] = child_base + __popc(imask & ((1u << k2) - 1u));
+                            st_t[slot] = __uint_as_float(m2 & ~7u);
                             sp += npush;
                         }
                     }
