@@ -52,6 +52,21 @@ struct SceneDevice {
     const float* texcoords;
 };
 
+// Global-address-space view of a device pointer.  Loads through it compile to
+// global_load; through a generic pointer that was itself read from memory (a
+// scene table entry) they compile to flat loads, which also wait on the LDS
+// counter.  (The host pass of the compiler sees plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PRT_GLOBAL __attribute__((address_space(1)))
+#else
+#define PRT_GLOBAL
+#endif
+template <class T>
+__device__ __forceinline__ const PRT_GLOBAL T* global_view(const T* p)
+{
+    return (const PRT_GLOBAL T*)p;
+}
+
 // Shading frame of a hit on original triangle `prim` (vertices p0, p1, p2,
 // hit coordinates (bu, bv)) for a ray along dir, from the per-vertex
 // attributes that are present: the full ShadingContext.
@@ -62,19 +77,22 @@ __device__ __forceinline__ Frame attribute_frame(const int32_t* idx, const float
     CornerAttributes a{};
     a.has_normals = normals != nullptr;
     a.has_texcoords = texcoords != nullptr;
-    const int i0 = idx[3 * (size_t)prim], i1 = idx[3 * (size_t)prim + 1], i2 = idx[3 * (size_t)prim + 2];
+    const PRT_GLOBAL int32_t* gi = global_view(idx);
+    const int i0 = gi[3 * (size_t)prim], i1 = gi[3 * (size_t)prim + 1], i2 = gi[3 * (size_t)prim + 2];
     if (a.has_normals) {
-        a.n0 = mk3(normals[3 * (size_t)i0], normals[3 * (size_t)i0 + 1], normals[3 * (size_t)i0 + 2]);
-        a.n1 = mk3(normals[3 * (size_t)i1], normals[3 * (size_t)i1 + 1], normals[3 * (size_t)i1 + 2]);
-        a.n2 = mk3(normals[3 * (size_t)i2], normals[3 * (size_t)i2 + 1], normals[3 * (size_t)i2 + 2]);
+        const PRT_GLOBAL float* gn = global_view(normals);
+        a.n0 = mk3(gn[3 * (size_t)i0], gn[3 * (size_t)i0 + 1], gn[3 * (size_t)i0 + 2]);
+        a.n1 = mk3(gn[3 * (size_t)i1], gn[3 * (size_t)i1 + 1], gn[3 * (size_t)i1 + 2]);
+        a.n2 = mk3(gn[3 * (size_t)i2], gn[3 * (size_t)i2 + 1], gn[3 * (size_t)i2 + 2]);
     }
     if (a.has_texcoords) {
-        a.u0 = texcoords[2 * (size_t)i0];
-        a.v0 = texcoords[2 * (size_t)i0 + 1];
-        a.u1 = texcoords[2 * (size_t)i1];
-        a.v1 = texcoords[2 * (size_t)i1 + 1];
-        a.u2 = texcoords[2 * (size_t)i2];
-        a.v2 = texcoords[2 * (size_t)i2 + 1];
+        const PRT_GLOBAL float* gt = global_view(texcoords);
+        a.u0 = gt[2 * (size_t)i0];
+        a.v0 = gt[2 * (size_t)i0 + 1];
+        a.u1 = gt[2 * (size_t)i1];
+        a.v1 = gt[2 * (size_t)i1 + 1];
+        a.u2 = gt[2 * (size_t)i2];
+        a.v2 = gt[2 * (size_t)i2 + 1];
     }
     return shading_frame(p0, p1, p2, dir, bu, bv, a);
 }

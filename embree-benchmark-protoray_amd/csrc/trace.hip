@@ -109,9 +109,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     int max_depth = pl.max_depth;
     float albedo = pl.albedo;
     // scene of the lane's current ray (per lane in the batch mode)
-    const Bvh8Node* nodes = sc.nodes;
-    const float* tris = sc.tris;
-    const int32_t* prim_ids = sc.prim_ids;
+    const PRT_GLOBAL Bvh8Node* nodes = global_view(sc.nodes);
+    const PRT_GLOBAL float* tris = global_view(sc.tris);
+    const PRT_GLOBAL int32_t* prim_ids = global_view(sc.prim_ids);
     int32_t scene_id = 0;  // batch mode: the lane's scene (for its shading attributes)
     float* sample_out = nullptr;
     // RGB throughput (RGB and adjoint modes); adjoint: per-channel product of
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             float t = -1.0f;
             f3 p0{}, p1{}, p2{};
             if (best_prim >= 0) {
-                const float* tp = tris + 9 * (size_t)best_rec;
+                const PRT_GLOBAL float* tp = tris + 9 * (size_t)best_rec;
                 p0 = mk3(tp[0], tp[1], tp[2]);
                 p1 = mk3(tp[3], tp[4], tp[5]);
                 p2 = mk3(tp[6], tp[7], tp[8]);
@@ -400,9 +400,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                             pixel = x + y * job.width;
                             const SceneDevice jsc = bl.scenes[job.scene];
                             scene_id = job.scene;
-                            nodes = jsc.nodes;
-                            tris = jsc.tris;
-                            prim_ids = jsc.prim_ids;
+                            nodes = global_view(jsc.nodes);
+                            tris = global_view(jsc.tris);
+                            prim_ids = global_view(jsc.prim_ids);
                             sample_out = bl.samples + job.sample_base +
                                          (size_t)sample * (size_t)(job.width * job.height) + pixel;
                             max_depth = job.max_depth;
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                 // with many candidate triangles do not hold up the wave
                 const uint32_t rec = pend_base + (uint32_t)__builtin_ctz(pend);
                 pend &= pend - 1;
-                const float* tp = tris + 9 * (size_t)rec;
+                const PRT_GLOBAL float* tp = tris + 9 * (size_t)rec;
                 const f3 p0 = mk3(tp[0], tp[1], tp[2]);
                 const f3 p1 = mk3(tp[3], tp[4], tp[5]);
                 const f3 p2 = mk3(tp[6], tp[7], tp[8]);
@@ -565,7 +565,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             } else if (pend2 == 0) {
 
                 // ---- visit one BVH8 node
-                const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
+                const PRT_GLOBAL uint4* np = reinterpret_cast<const PRT_GLOBAL uint4*>(nodes + node);
                 const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
                 if (kCount) ++n_nodes;
                 const int ex = (int)(int8_t)(w0.w & 0xffu);
@@ -791,7 +791,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                     bool hitc[4] = {false, false, false, false};
                     uint4 w0{}, w1{};
                     if (node_step) {
-                        const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
+                        const PRT_GLOBAL uint4* np = reinterpret_cast<const PRT_GLOBAL uint4*>(nodes + node);
                         w0 = np[0];
                         w1 = np[1];
                         const uint4 w2 = np[2], w3 = np[3], w4 = np[4];
@@ -926,7 +926,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                         uint32_t crec = 0;
                         if (mj != 0) {
                             const uint32_t rec = pend_base + (uint32_t)__builtin_ctz(mj);
-                            const float* tp = tris + 9 * (size_t)rec;
+                            const PRT_GLOBAL float* tp = tris + 9 * (size_t)rec;
                             const f3 p0 = mk3(tp[0], tp[1], tp[2]);
                             const f3 p1 = mk3(tp[3], tp[4], tp[5]);
                             const f3 p2 = mk3(tp[6], tp[7], tp[8]);
@@ -967,7 +967,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                         }
                         active = false;
                     } else {
-                        const float* tp = tris + 9 * (size_t)best_rec;
+                        const PRT_GLOBAL float* tp = tris + 9 * (size_t)best_rec;
                         const f3 p0 = mk3(tp[0], tp[1], tp[2]);
                         const f3 p1 = mk3(tp[3], tp[4], tp[5]);
                         const f3 p2 = mk3(tp[6], tp[7], tp[8]);
