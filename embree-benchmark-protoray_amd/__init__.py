@@ -139,6 +139,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "prt_hip_set_pipeline": ([P, ctypes.c_int], ctypes.c_int),
         "prt_hip_set_bvh_collapse": ([P, ctypes.c_int, ctypes.c_float], ctypes.c_int),
         "prt_hip_set_tail": ([P, ctypes.c_int], ctypes.c_int),
+        "prt_hip_set_lookahead": ([P, ctypes.c_int], ctypes.c_int),
         "prt_hip_accum_read": ([P, P], ctypes.c_int),
         "prt_hip_frame_read_bgr8": ([P, P], ctypes.c_int),
         "prt_hip_debug_raygen": ([P, ctypes.c_int, P, P], ctypes.c_int),
@@ -366,6 +367,13 @@ class HipDevice:
         if mode not in ids:
             raise ValueError("mode must be 'cooperative' or 'lanes'")
         self._check(self.lib.prt_hip_set_tail(self.ctx, ids[mode]), "set_tail")
+
+    def set_lookahead(self, enable: bool = True) -> None:
+        """Frame lookahead of the fused diffuse renderer (default on): idle
+        lanes at the end of a frame start the next pass's paths, which the next
+        render call keeps if it renders that pass with the same inputs.  Images
+        and ray counts per call do not change."""
+        self._check(self.lib.prt_hip_set_lookahead(self.ctx, 1 if enable else 0), "set_lookahead")
 
     def set_pipeline(self, pipeline: str) -> None:
         """'fused' (one persistent path launch per frame) or 'wavefront' (per-depth launches)."""

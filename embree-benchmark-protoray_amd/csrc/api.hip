@@ -8,6 +8,7 @@
 // read of the per-depth counters for the ray statistics.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -229,7 +230,8 @@ int prt_hip_destroy(prt_hip_ctx* ctx)
     void* ptrs[] = {ctx->d_spill,
                     ctx->d_accum, ctx->d_bgr8, ctx->d_rays[0], ctx->d_rays[1], ctx->d_hits,
                     ctx->d_pix[0], ctx->d_pix[1], ctx->d_rng[0], ctx->d_rng[1], ctx->d_counters,
-                    ctx->d_io_rays, ctx->d_io_hits};
+                    ctx->d_io_rays, ctx->d_io_hits, ctx->d_look, ctx->d_look_side, ctx->d_look_q[0],
+                    ctx->d_look_q[1], ctx->d_look_flags};
     for (void* p : ptrs) free_ptr(p);
     for (hipEvent_t e : ctx->kev)
         if (e) (void)hipEventDestroy(e);
@@ -261,6 +263,7 @@ int prt_hip_scene_upload(prt_hip_ctx* ctx, const int32_t* idx3, int64_t tri_coun
     for (int64_t i = 0; i < 3 * tri_count; ++i)
         if (idx3[i] < 0 || idx3[i] >= vertex_count)
             return set_error(ctx, "scene_upload: vertex index out of range");
+    ctx->scene_gen += 1;
     free_ptr(ctx->scene.d_idx);
     free_ptr(ctx->scene.d_pos);
     free_ptr(ctx->scene.d_normals);
@@ -289,6 +292,7 @@ int prt_hip_scene_attributes(prt_hip_ctx* ctx, const float* normals3, const floa
 {
     if (use_device(ctx)) return 1;
     if (!ctx->scene.d_idx) return set_error(ctx, "scene_attributes: no scene uploaded");
+    ctx->scene_gen += 1;
     return set_attributes(ctx, ctx->scene, normals3, texcoords2);
 }
 
@@ -306,6 +310,7 @@ int prt_hip_bvh_build(prt_hip_ctx* ctx, prt_build_stats* out_stats)
 {
     if (use_device(ctx)) return 1;
     if (!ctx->scene.d_idx) return set_error(ctx, "bvh_build: no scene uploaded");
+    ctx->scene_gen += 1;
     if (build_bvh8(ctx, ctx->scene, out_stats)) return 1;
     return ensure_trace_resources(ctx);
 }
@@ -496,6 +501,67 @@ int prt_hip_set_tail(prt_hip_ctx* ctx, int mode)
     return 0;
 }
 
+int prt_hip_set_lookahead(prt_hip_ctx* ctx, int enable)
+{
+    if (!ctx) return 1;
+    ctx->lookahead = enable ? 1 : 0;
+    ctx->look_prev_on = false;
+    return 0;
+}
+
+namespace {
+
+// Buffers of the frame lookahead: the state, one side entry per pixel and two
+// queues (read by this call, written for the next) of one path per lane.
+int ensure_look_resources(prt_hip_ctx* ctx)
+{
+    const int64_t pixels = (int64_t)ctx->width * ctx->height;
+    const uint32_t cap = (uint32_t)ctx->trace_grid * (uint32_t)kTraceBlock;
+    bool fresh = false;
+    if (!ctx->d_look) {
+        PRT_TRY(ctx, hipMalloc(&ctx->d_look, sizeof(LookState)));
+        PRT_TRY(ctx, hipMemsetAsync(ctx->d_look, 0, sizeof(LookState), ctx->stream));
+        fresh = true;
+    }
+    if (pixels != ctx->look_side_len) {
+        free_ptr(ctx->d_look_side);
+        ctx->d_look_side = nullptr;
+        ctx->look_side_len = 0;
+        PRT_TRY(ctx, hipMalloc(&ctx->d_look_side, sizeof(uint2) * pixels));
+        PRT_TRY(ctx, hipMemsetAsync(ctx->d_look_side, 0, sizeof(uint2) * pixels, ctx->stream));
+        ctx->look_side_len = pixels;
+        fresh = true;
+    }
+    if (cap > ctx->look_q_cap) {
+        for (uint4*& q : ctx->d_look_q) {
+            free_ptr(q);
+            q = nullptr;
+        }
+        free_ptr(ctx->d_look_flags);
+        ctx->d_look_flags = nullptr;
+        ctx->look_q_cap = 0;
+        for (uint4*& q : ctx->d_look_q) PRT_TRY(ctx, hipMalloc(&q, 3 * sizeof(uint4) * (size_t)cap));
+        const size_t flag_bytes = sizeof(uint32_t) * kLookFlagStride * (size_t)ctx->trace_grid;
+        PRT_TRY(ctx, hipMalloc(&ctx->d_look_flags, flag_bytes));
+        PRT_TRY(ctx, hipMemsetAsync(ctx->d_look_flags, 0, flag_bytes, ctx->stream));
+        ctx->look_q_cap = cap;
+        fresh = true;
+    }
+    if (fresh) ctx->look_prev_on = false;
+    return 0;
+}
+
+// The previous call's lookahead work belongs to this call: the same inputs and
+// the next pass.
+bool look_follows(const prt_hip_ctx::LookKey& prev, const prt_hip_ctx::LookKey& cur)
+{
+    return prev.scene_gen == cur.scene_gen && std::memcmp(&prev.cam, &cur.cam, sizeof(prt_camera)) == 0 &&
+           prev.width == cur.width && prev.height == cur.height && prev.max_depth == cur.max_depth &&
+           prev.grid == cur.grid && prev.albedo_bits == cur.albedo_bits && prev.pass + 1 == cur.pass;
+}
+
+}  // namespace
+
 int prt_hip_set_pipeline(prt_hip_ctx* ctx, int pipeline)
 {
     if (!ctx) return 1;
@@ -528,6 +594,22 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
     const int tiles_x = (ctx->width + 7) / 8;
     const uint32_t entries = (uint32_t)frame_entries(ctx);
 
+    const bool look_on = fused && renderer == PRT_RENDERER_DIFFUSE && !fast && !counting && ctx->lookahead != 0;
+    prt_hip_ctx::LookKey key{};
+    if (look_on) {
+        if (ensure_look_resources(ctx)) return 1;
+        key.scene_gen = ctx->scene_gen;
+        key.cam = ctx->cam;
+        key.width = ctx->width;
+        key.height = ctx->height;
+        key.max_depth = last;
+        key.pass = ctx->pass;
+        key.grid = ctx->trace_grid;
+        std::memcpy(&key.albedo_bits, &albedo, sizeof(float));
+    }
+    const bool look_valid = look_on && ctx->look_prev_on && look_follows(ctx->look_prev, key);
+    ctx->look_prev_on = false;  // set again once this call has succeeded
+
     PRT_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (fused) {
         PRT_TRY(ctx, launch_frame_begin(ctx, entries));
@@ -548,6 +630,19 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
         p.stats = ctx->d_stats;
         p.fast = fast;
         p.coop = ctx->tail_coop;
+        if (look_on) {
+            ctx->look_gen += 1;
+            PRT_TRY(ctx, launch_look_begin(ctx, look_valid, entries));
+            p.look = ctx->d_look;
+            p.side = ctx->d_look_side;
+            p.q_in = ctx->d_look_q[(ctx->look_gen - 1) & 1];
+            p.q_out = ctx->d_look_q[ctx->look_gen & 1];
+            p.q_cap = ctx->look_q_cap;
+            p.look_gen = ctx->look_gen;
+            p.look_flags = ctx->d_look_flags;
+            p.path_counter = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ctx->d_look) +
+                                                         offsetof(LookState, counter));
+        }
         if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[0], ctx->stream));
         PRT_TRY(ctx, launch_paths(ctx, p, renderer, counting));
         if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[1], ctx->stream));
@@ -584,9 +679,15 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
     const uint32_t* counters = rb.counters;
     const unsigned long long* st = rb.st;
     PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    if (counters[32]) return set_error(ctx, "render: traversal stack overflow");
+    if (counters[32])
+        return set_error(ctx, (counters[32] & 2u) ? "render: lookahead queue overflow"
+                                                  : "render: traversal stack overflow");
     float ms = 0.0f;
     PRT_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (look_on) {
+        ctx->look_prev = key;
+        ctx->look_prev_on = true;
+    }
     ctx->pass += 1;
     if (out) {
         std::memset(out, 0, sizeof(*out));

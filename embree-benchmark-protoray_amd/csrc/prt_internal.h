@@ -154,6 +154,8 @@ struct BatchTuning {
     int refill = 32;
 };
 
+struct LookState;
+
 }  // namespace prt
 
 struct prt_hip_ctx {
@@ -222,6 +224,24 @@ struct prt_hip_ctx {
     // cooperative tail of the fused diffuse frame (prt_hip_set_tail)
     int tail_coop = 1;
 
+    // frame lookahead of the fused diffuse frame (prt_hip_set_lookahead)
+    int lookahead = 1;
+    uint64_t scene_gen = 0;             // bumped by every change of the plain scene
+    prt::LookState* d_look = nullptr;
+    uint2* d_look_side = nullptr;       // one entry per pixel
+    int64_t look_side_len = 0;
+    uint4* d_look_q[2] = {nullptr, nullptr};
+    uint32_t look_q_cap = 0;
+    uint32_t* d_look_flags = nullptr;   // one stop flag per block of the launch
+    uint32_t look_gen = 0;
+    struct LookKey {
+        uint64_t scene_gen;
+        prt_camera cam;
+        int width, height, max_depth, pass, grid;
+        uint32_t albedo_bits;
+    } look_prev{};
+    bool look_prev_on = false;
+
     // BVH8 collapse (prt_hip_set_bvh_collapse)
     int bvh_collapse = PRT_COLLAPSE_SAH;
     float bvh_prim_cost = 1.0f;     // triangle test cost relative to a wide-node visit
@@ -243,6 +263,26 @@ struct TraceLaunch {
     unsigned long long* stats;  // nullptr unless instrumented
 };
 // Fused path launch (one frame, one sample per pixel)
+// Frame lookahead of the fused diffuse renderer: device state that one render
+// call hands to the next.  A call renders frame p: first the paths of frame p
+// that the previous call suspended, then the items of frame p it did not
+// start; once those are all handed out, idle lanes start items of frame p + 1.
+// When the last path of frame p finishes, next-frame paths stop at their next
+// bounce and are queued for the next call; those that finished already wait
+// in a per-pixel side buffer, tagged with the call that wrote them.
+struct LookState {
+    uint32_t counter;       // work counter over this frame's virtual items [0, R + entries - S)
+    uint32_t spec_counter;  // work counter over the next frame's items
+    uint32_t resume_in;     // R: suspended paths of this frame in the input queue
+    uint32_t started;       // S: items of this frame the previous call started
+    uint32_t target;        // paths of this frame left: R + (entries - S)
+    uint32_t live;          // waves that hold, or can still take, paths of this frame
+    uint32_t resume_out;    // next-frame paths suspended by this call
+    uint32_t gen;           // tag of this call's side samples and stop flags
+    unsigned long long spec_rays;  // rays of the next frame traced by this call
+};
+constexpr int kLookFlagStride = 16;  // u32 per block stop flag (one 64 B line each)
+
 struct PathLaunch {
     Camera cam;
     int width, height, tiles_x;
@@ -268,12 +308,24 @@ struct PathLaunch {
     // diffuse frames: a wave with at most 8 paths left traces them
     // cooperatively (0: never)
     int coop;
+    // diffuse frames: frame lookahead (nullptr: off)
+    LookState* look;
+    uint2* side;          // per pixel: (next-frame sample bits, tag)
+    const uint4* q_in;    // suspended paths of this frame, 3 uint4 each
+    uint4* q_out;         // suspended paths of the next frame
+    uint32_t q_cap;       // capacity of q_out in paths (>= lanes of the launch)
+    uint32_t look_gen;    // this call's tag
+    uint32_t* look_flags; // per block (kLookFlagStride apart): look_gen once this frame is finished
 };
 int ensure_trace_resources(prt_hip_ctx* ctx);
 hipError_t launch_trace(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumented);
 hipError_t launch_occluded(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumented);
 // renderer: PRT_RENDERER_PRIMARY, _DIFFUSE or _DIFFUSE2
 hipError_t launch_paths(prt_hip_ctx* ctx, const PathLaunch& p, int renderer, bool instrumented);
+// frame lookahead: roll the state to a new call (valid: the previous call
+// rendered the previous pass with the same inputs and lookahead on), then add
+// the side samples of the previous call to the accumulator
+hipError_t launch_look_begin(prt_hip_ctx* ctx, bool valid, uint32_t entries);
 hipError_t launch_paths_rgb(prt_hip_ctx* ctx, const PathLaunch& p, bool instrumented);
 hipError_t launch_adjoint(prt_hip_ctx* ctx, const PathLaunch& p, bool instrumented);
 // max_blocks_per_cu != nullptr: only query the variant's occupancy limit

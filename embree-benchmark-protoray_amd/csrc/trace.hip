@@ -138,8 +138,49 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     // 8 paths left, the wave traces them cooperatively, 8 lanes per path
     // (see the cooperative phase after the main loop)
     constexpr bool kCoop = kMode == kModeDiffuse;
-    const bool coop_on = kCoop && pl.coop != 0;
+    const bool coop_on = kCoop && pl.coop != 0 && pl.look == nullptr;
     bool cont = false;  // the path's next ray waits in (o, d) for the cooperative phase
+
+    // Frame lookahead (kModeDiffuse, LookState).  This frame's virtual items,
+    // from LookState::counter: [0, R) resume the suspended paths of this
+    // frame, [R, fresh_end) are its items S..entries-1.  Once a wave has seen
+    // them all handed out and holds no path of this frame, it takes items of
+    // the next frame from LookState::spec_counter; waves that hold paths of
+    // this frame take nothing more (they stay sparse and issue first).
+    // Every wave starts registered in LookState::live and leaves
+    // it once, when it has seen every item of this frame handed out and holds
+    // no path of this frame; the last to leave writes this call's tag into
+    // each block's stop flag.  After the stop no more items are taken and
+    // next-frame paths leave at their next bounce.
+    constexpr bool kLookMode = kMode == kModeDiffuse;
+    LookState* const look = kLookMode ? pl.look : nullptr;
+    uint32_t lk_R = 0, lk_S = 0, lk_fresh_end = 0, lk_target = 0;
+    if (kLookMode && look) {
+        lk_R = look->resume_in;
+        lk_S = look->started;
+        lk_target = look->target;
+        lk_fresh_end = lk_R + (count - lk_S);
+    }
+    bool stop = kLookMode && look && lk_target == 0;  // wave-uniform: this frame is finished
+    bool fresh_out = false;  // wave-uniform: every item of this frame is handed out
+    bool live = kLookMode && look;  // wave-uniform: registered in LookState::live
+    bool spec = false;       // the lane's path belongs to the next frame
+    bool suspend = false;    // the lane's next-frame path waits in (o, d) for the queue
+    uint32_t poll_steps = 0;
+    uint32_t* const stop_flag = kLookMode && look ? pl.look_flags + (size_t)blockIdx.x * kLookFlagStride : nullptr;
+    // leaves LookState::live; the last wave to leave raises every stop flag
+    auto look_leave = [&]() {
+        uint32_t old = 0;
+        if (lane == 0) old = atomicSub(&look->live, 1u);
+        old = __shfl(old, 0, 64);
+        if (old == 1u)
+            for (uint32_t b = (uint32_t)lane; b < gridDim.x; b += 64u)
+                __hip_atomic_store(pl.look_flags + (size_t)b * kLookFlagStride, pl.look_gen, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        live = false;
+    };
+    unsigned long long n_spec_rays = 0;
+    if (stop) exhausted = true;
 
     auto start_ray = [&](f3 org, f3 dir, float near, float far) {
         o = org;
@@ -189,6 +230,30 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     };
 
     for (;;) {
+        // lookahead: a wave that holds paths of this frame takes no next-frame
+        // items once this frame's are handed out, and issues ahead of the
+        // waves that run next-frame paths, so the frame's last paths keep the
+        // pace they would have alone
+        bool hold = false;
+        if (kLookMode && look) {
+            hold = fresh_out && __ballot(active && !spec) != 0;
+            if (live && fresh_out && !hold) look_leave();
+            if (!live && !stop) {
+                const uint32_t f = __hip_atomic_load(stop_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                stop = __builtin_amdgcn_readfirstlane(f) == pl.look_gen;
+                if (stop) exhausted = true;
+            }
+            // after the stop only next-frame paths are left: those in the
+            // middle of a ray leave now and trace that ray again next call
+            // (its origin, direction and path state are what a bounce leaves)
+            if (stop && tracing) {
+                suspend = true;
+                tracing = false;
+                active = false;
+            }
+            if (hold) __builtin_amdgcn_s_setprio(3);
+            else __builtin_amdgcn_s_setprio(0);
+        }
         // the wave goes cooperative once out of work, with at most 8 paths
         // left and all of them between rays
         bool go_coop = false;
@@ -232,7 +297,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                 break;
             }
             if constexpr (kCount && kMode != kModeBatch) atomicAdd(&depth_hist[depth], 1u);
-            ++n_rays;
+            if (kLookMode && spec) ++n_spec_rays;
+            else ++n_rays;
             float4* acc = kMode == kModeBatch ? nullptr : pl.accum + pixel;
             if (kMode == kModeDiffuse2) {
                 // extension ray done (render/diffuse2_integrator_single.h:52-85)
@@ -302,6 +368,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                     const float cg = best_prim < 0 ? lg : 0.0f, cb = best_prim < 0 ? lb : 0.0f;
                     const float4 a = *acc;
                     *acc = make_float4(a.x + c, a.y + cg, a.z + cb, a.w + 1.0f);
+                } else if (kLookMode && spec) {
+                    pl.side[pixel] = make_uint2(__float_as_uint(c), pl.look_gen);
                 } else {
                     const float4 a = pl.fast ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : *acc;
                     *acc = make_float4(a.x + c, a.y + c, a.z + c, a.w + 1.0f);
@@ -344,6 +412,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             f3 no, nd;
             diffuse_bounce(o, d, t, fr, s1, s2, no, nd);
             ++depth;
+            if (kLookMode && spec && stop) {
+                o = no;
+                d = nd;
+                suspend = true;
+                active = false;
+                break;
+            }
             if (kCoop && go_coop) {
                 o = no;
                 d = nd;
@@ -354,20 +429,46 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
         } while (0);
         if (kCoop && go_coop) break;
 
+        // ---- lookahead: queue the next-frame paths that stopped at a bounce
+        if (kLookMode && look) {
+            const uint64_t sm = __ballot(suspend);
+            if (sm != 0) {
+                const int leader = __ffsll((unsigned long long)sm) - 1;
+                uint32_t qb = 0;
+                if (lane == leader) qb = atomicAdd(&look->resume_out, (uint32_t)__popcll(sm));
+                qb = __shfl(qb, leader, 64);
+                if (suspend) {
+                    const uint32_t slot = qb + (uint32_t)__popcll(sm & lane_lt);
+                    if (slot < pl.q_cap) {
+                        uint4* q = pl.q_out + 3 * (size_t)slot;
+                        q[0] = make_uint4(__float_as_uint(o.x), __float_as_uint(o.y), __float_as_uint(o.z),
+                                          (uint32_t)pixel);
+                        q[1] = make_uint4(__float_as_uint(d.x), __float_as_uint(d.y), __float_as_uint(d.z), rng);
+                        q[2] = make_uint4((uint32_t)depth, __float_as_uint(lw), 0u, 0u);
+                    } else {
+                        atomicOr(err, 2u);
+                    }
+                    suspend = false;
+                }
+            }
+        }
+
         // ---- refill idle lanes (wave-uniform branch)
         const uint64_t idle = __ballot(!active);
         const int n_idle = __popcll(idle);
-        if (!exhausted && n_idle >= refill) {
+        if (!exhausted && !hold && n_idle >= refill) {
             const int leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t* ctr = kRayQuery ? tl.fetch_ptr : (kMode == kModeBatch ? bl.work_counter : pl.path_counter);
+            uint32_t lim = count;
+            const bool spec_claim = kLookMode && look && fresh_out;  // next-frame items
+            if (kLookMode && look && !spec_claim) lim = lk_fresh_end;
+            if (spec_claim) ctr = &look->spec_counter;
             uint32_t base = 0;
-            if (lane == leader)
-                base = atomicAdd(kRayQuery ? tl.fetch_ptr
-                                                     : (kMode == kModeBatch ? bl.work_counter : pl.path_counter),
-                                 (uint32_t)n_idle);
+            if (lane == leader) base = atomicAdd(ctr, (uint32_t)n_idle);
             base = __shfl(base, leader, 64);
             if (!active) {
                 const uint32_t my = base + (uint32_t)__popcll(idle & lane_lt);
-                if (my < count) {
+                if (my < lim) {
                     if (kRayQuery) {
                         ray_idx = my;
                         const float4 r0 = tl.rays[2 * (size_t)my];
@@ -420,11 +521,29 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                             lw = 1.0f;
                             active = true;
                         }
+                    } else if (kLookMode && look && !spec_claim && my < lk_R) {
+                        // a path of this frame that the previous call stopped
+                        // at a bounce: its next ray
+                        const uint4* q = pl.q_in + 3 * (size_t)my;
+                        const uint4 e0 = q[0], e1 = q[1], e2 = q[2];
+                        pixel = (int32_t)e0.w;
+                        rng = e1.w;
+                        depth = (int)e2.x;
+                        lw = __uint_as_float(e2.y);
+                        spec = false;
+                        start_ray(mk3(as_f(e0.x), as_f(e0.y), as_f(e0.z)), mk3(as_f(e1.x), as_f(e1.y), as_f(e1.z)),
+                                  0.0f, 3.402823466e38f);
+                        active = true;
                     } else {
                         // camera ray of path `my`: 8x8 tiles, Morton order inside
                         // a tile (image/morton.cpp:22), as the wavefront raygen
                         uint32_t path = my;
                         path_pass = pl.pass;
+                        if (kLookMode && look) {
+                            spec = spec_claim;
+                            path = spec ? my : lk_S + (my - lk_R);
+                            path_pass = pl.pass + (spec ? 1 : 0);
+                        }
                         if (kMode == kModeAdjoint) {
                             const uint32_t sample = my / pl.entries;
                             path = my - sample * pl.entries;
@@ -467,11 +586,16 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                     }
                 }
             }
-            if ((uint64_t)base + (uint64_t)n_idle >= count) {
-                exhausted = true;
-                if (kCount) t_exhausted = wall_clock64();
+            if ((uint64_t)base + (uint64_t)n_idle >= lim) {
+                if (kLookMode && look && !spec_claim) {
+                    fresh_out = true;
+                } else {
+                    exhausted = true;
+                    if (kCount) t_exhausted = wall_clock64();
+                }
             }
         }
+
         const uint64_t busy = __ballot(active);
         if (busy == 0) {
             if (exhausted) break;
@@ -482,7 +606,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
         // refill lanes once no more than 64 - refill trace; after the work
         // counter ran out, shade once half of the remaining lanes wait
         const int n_act = __popcll(busy);
-        const int keep = !exhausted ? 64 - refill
+        const int keep = !(exhausted || hold) ? 64 - refill
                                     : (coop_on && n_act <= 8 ? 0 : n_act - max(1, min(refill, n_act >> 1)));
         while (tracing) {
             if (kCount) {
@@ -711,8 +835,20 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             }
 
             if ((int)__popcll(__ballot(1)) <= keep) break;
+            // lookahead: look at the stop flag every 8 steps (its load is
+            // waited for with the next node's)
+            if (kLookMode && look && !live && !stop && (++poll_steps & 7u) == 0) {
+                const uint32_t f = __hip_atomic_load(stop_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (__builtin_amdgcn_readfirstlane(f) == pl.look_gen) {
+                    stop = true;
+                    exhausted = true;
+                    break;
+                }
+            }
         }
     }
+
+    if (kLookMode && look && live) look_leave();
 
     // ---- cooperative tail (kModeDiffuse).  Each of the wave's remaining
     // paths (at most 8) moves to a group of G = 8 lanes that hold its ray
@@ -1010,6 +1146,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             // drain window: earliest exhaustion, latest exit (wall clock ticks)
             if (t_exhausted) atomicMin(&st[9], t_exhausted);
             atomicMax(&st[10], (unsigned long long)wall_clock64());
+        }
+    }
+    if constexpr (kLookMode) {
+        if (look) {
+            unsigned long long r = n_spec_rays;
+            for (int off = 32; off > 0; off >>= 1) r += __shfl_down(r, off, 64);
+            if (lane == 0 && r) atomicAdd(&look->spec_rays, r);
         }
     }
     if constexpr (!kRayQuery) {

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PRT_HIP_ABI_VERSION 3
+#define PRT_HIP_ABI_VERSION 4
 
 typedef struct prt_hip_ctx prt_hip_ctx;
 
@@ -237,6 +237,15 @@ int prt_hip_set_pipeline(prt_hip_ctx* ctx, int pipeline);
 #define PRT_TAIL_LANES       0
 #define PRT_TAIL_COOPERATIVE 1
 int prt_hip_set_tail(prt_hip_ctx* ctx, int mode);
+/* Frame lookahead of the fused diffuse renderer (no counterpart in the
+ * reference; default on).  While the last paths of frame p run, idle lanes
+ * start paths of frame p + 1; the call returns when frame p is complete, and
+ * the next call, if it renders pass p + 1 with the same scene, camera, frame
+ * size, max depth and albedo, keeps that work (else drops it).  The
+ * accumulator after each call and its ray count are those of rendering the
+ * frames one by one.  Off for the primary, diffuse2 and Fast renderers, the
+ * wavefront pipeline and under PRT_PROFILE_COUNTERS. */
+int prt_hip_set_lookahead(prt_hip_ctx* ctx, int enable);
 
 /* --- frame buffer read-back -------------------------------------------- */
 /* Raw accumulation buffer, float4 (r, g, b, weight) per pixel, row-major. */

@@ -299,7 +299,9 @@ def test_render_diffuse_bit_exact(dev, orc, scenes, pipeline):
 def test_frame_tail_modes_bit_exact(dev, orc, scenes, tail):
     """The fused frame's tail, traced with 8 lanes per remaining path or one,
     gives the oracle's image and ray count (spheres at 480x270, depth 8, where
-    many paths are still bouncing when the frame's work runs out)."""
+    many paths are still bouncing when the frame's work runs out).  The tail
+    modes apply with the frame lookahead off (with it on, idle lanes take the
+    next pass's paths instead)."""
     mesh = scenes.scene_spheres(1002, count=96, subdiv=3)
     _load(dev, mesh)
     w, h = 480, 270
@@ -307,10 +309,12 @@ def test_frame_tail_modes_bit_exact(dev, orc, scenes, tail):
     dev.set_camera(cam)
     dev.init_frame(w, h)
     dev.set_tail(tail)
+    dev.set_lookahead(False)
     try:
         rays = sum(dev.render("diffuseHip", max_depth=8, albedo=0.7)["ray"] for _ in range(2))
     finally:
         dev.set_tail("cooperative")
+        dev.set_lookahead(True)
     ref = orc.Scene(mesh.idx, mesh.pos).render(cam, w, h, spp=2, max_depth=8, albedo=0.7, renderer=1, numerics=0)
     got = dev.accum()
     assert np.array_equal(got, ref["accum"]), f"{int((got != ref['accum']).any(-1).sum())} pixels differ"

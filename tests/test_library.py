@@ -15,7 +15,7 @@ def test_library_exports_every_declared_symbol(prt):
     lib = prt.load_library()
     missing = [n for n in prt.declared_symbols() if not hasattr(lib, n)]
     assert not missing, f"missing exports: {missing}"
-    assert lib.prt_hip_abi_version() == 3
+    assert lib.prt_hip_abi_version() == 4
 
 
 def test_no_context_without_gpu_is_an_error_not_a_crash(prt):
