@@ -142,8 +142,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     bool cont = false;  // the path's next ray waits in (o, d) for the cooperative phase
 
     // Frame lookahead (kModeDiffuse, LookState).  This frame's virtual items,
-    // from LookState::counter: [0, R) resume the suspended paths of this
-    // frame, [R, fresh_end) are its items S..entries-1.  Once a wave has seen
+    // from LookState::counter: [0, F) are its items S..entries-1 (F =
+    // entries - S), [F, fresh_end) resume the R paths the previous call
+    // suspended (they are part way through, so they go last).  Once a wave has seen
     // them all handed out and holds no path of this frame, it takes items of
     // the next frame from LookState::spec_counter; waves that hold paths of
     // this frame take nothing more (they stay sparse and issue first).
@@ -151,15 +152,16 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     // it once, when it has seen every item of this frame handed out and holds
     // no path of this frame; the last to leave writes this call's tag into
     // each block's stop flag.  After the stop no more items are taken and
-    // next-frame paths leave at their next bounce.
+    // next-frame paths leave for the queue: at a bounce, or in the middle of
+    // a ray, which the next call then traces again from its origin.
     constexpr bool kLookMode = kMode == kModeDiffuse;
     LookState* const look = kLookMode ? pl.look : nullptr;
-    uint32_t lk_R = 0, lk_S = 0, lk_fresh_end = 0, lk_target = 0;
+    uint32_t lk_F = 0, lk_S = 0, lk_fresh_end = 0, lk_target = 0;
     if (kLookMode && look) {
-        lk_R = look->resume_in;
+        lk_F = count - look->started;
         lk_S = look->started;
         lk_target = look->target;
-        lk_fresh_end = lk_R + (count - lk_S);
+        lk_fresh_end = lk_F + look->resume_in;
     }
     bool stop = kLookMode && look && lk_target == 0;  // wave-uniform: this frame is finished
     bool fresh_out = false;  // wave-uniform: every item of this frame is handed out
@@ -521,10 +523,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                             lw = 1.0f;
                             active = true;
                         }
-                    } else if (kLookMode && look && !spec_claim && my < lk_R) {
+                    } else if (kLookMode && look && !spec_claim && my >= lk_F) {
                         // a path of this frame that the previous call stopped
                         // at a bounce: its next ray
-                        const uint4* q = pl.q_in + 3 * (size_t)my;
+                        const uint4* q = pl.q_in + 3 * (size_t)(my - lk_F);
                         const uint4 e0 = q[0], e1 = q[1], e2 = q[2];
                         pixel = (int32_t)e0.w;
                         rng = e1.w;
@@ -541,7 +543,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                         path_pass = pl.pass;
                         if (kLookMode && look) {
                             spec = spec_claim;
-                            path = spec ? my : lk_S + (my - lk_R);
+                            path = spec ? my : lk_S + my;
                             path_pass = pl.pass + (spec ? 1 : 0);
                         }
                         if (kMode == kModeAdjoint) {
