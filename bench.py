@@ -13,7 +13,11 @@ Prints ONE JSON line.  `value` is wall-clock Mrays/s over the K timed frames
 with the scene, BVH and path state resident in HBM.  The default pipeline is
 the fused persistent path kernel (raygen, closest hit, cosine bounce and
 accumulation in one launch per frame), which is the dominant and only kernel
-of a frame.  `roofline` is for it: algorithmic bytes per frame
+of a frame, with the frame lookahead on (the library's default: while a
+frame's last paths run, idle lanes start the next pass's paths, which the next
+call keeps; each call's accumulator and ray count stay those of its own frame).
+The same frames with the lookahead off are timed beside it
+(`no_lookahead_mrays`).  `roofline` is for the path kernel: algorithmic bytes per frame
 rays * (80 * nodes/ray + 36 * tris/ray) + 32 * paths (BVH8 nodes and triangle
 records the traversal fetches, the accumulator read + write per path), with
 nodes/tris per ray from an instrumented replay of the same frames, divided by
@@ -92,7 +96,8 @@ def cpu_baseline(scenes, mesh, cfg, cam, frames: int) -> dict:
                       f"{r['rays']} rays in {r['seconds']:.2f} s"}
 
 
-def bench_batched(pkg, scenes, dev, mesh, cam, cfg, args, plain_mrays: float, wavefront_mrays: float) -> dict:
+def bench_batched(pkg, scenes, dev, mesh, cam, cfg, args, plain_mrays: float, wavefront_mrays: float,
+                  no_look_mrays: float) -> dict:
     """Batched form.  (1) spheres_1080p as one job of spp = steps: the same
     frames as the plain run (passes warmup..warmup+steps-1) in one persistent
     launch per render; (2) the 32-scene workload (triangles log-uniform
@@ -143,6 +148,7 @@ def bench_batched(pkg, scenes, dev, mesh, cam, cfg, args, plain_mrays: float, wa
     dev.batch_reset()
     return {"spheres_1080p_mrays": round(c2, 2), "spheres_1080p_spp_per_launch": args.steps,
             "ratio_vs_plain_fused": round(c2 / plain_mrays, 3),
+            "ratio_vs_plain_fused_no_lookahead": round(c2 / no_look_mrays, 3),
             "ratio_vs_plain_wavefront": round(c2 / wavefront_mrays, 3),
             "spheres_1080p_tuning": chosen2,
             "scenes32_mrays": round(c3, 2), "scenes32_triangles": tris, "scenes32_paths": st3[0]["paths"],
@@ -318,6 +324,18 @@ def main() -> int:
     log(f"timed {args.steps} frames: {rays} rays in {wall * 1e3:.1f} ms wall ({dev_ms:.1f} ms device) "
         f"-> {value:.1f} Mrays/s")
 
+    # the same frames with the frame lookahead off (each call starts its frame
+    # from nothing and ends with its own tail)
+    dev.set_lookahead(False)
+    dev.set_pass(args.warmup)
+    t2 = time.perf_counter()
+    nstats = [dev.render("diffuseHip", max_depth=depth, albedo=albedo) for _ in range(args.steps)]
+    no_look_wall = time.perf_counter() - t2
+    dev.set_lookahead(True)
+    assert sum(s["ray"] for s in nstats) == rays, "lookahead changed the ray count"
+    no_look = rays / no_look_wall / 1e6
+    log(f"same frames without the frame lookahead: {no_look:.1f} Mrays/s")
+
     # replay of the same passes: per-launch device time, then traversal counters
     def replay(timing: bool, counters: bool) -> list:
         dev.set_pass(args.warmup)
@@ -369,7 +387,7 @@ def main() -> int:
     # batched form, config 2: the 16 spp of the same frame sequence in one launch
     batched = None
     if not args.no_batched:
-        batched = bench_batched(pkg, scenes, dev, mesh, cam, cfg, args, value, wrays / wwall / 1e6)
+        batched = bench_batched(pkg, scenes, dev, mesh, cam, cfg, args, value, wrays / wwall / 1e6, no_look)
 
     traffic = measured_traffic()
     out = {
@@ -394,6 +412,7 @@ def main() -> int:
                      "bytes_per_ray": round(kbytes / rays, 1),
                      "avg_launch_ms": round(kernel_ms / args.steps, 4)},
         "device_mrays": round(rays / (dev_ms * 1e-3) / 1e6, 2),
+        "no_lookahead_mrays": round(no_look, 2),
         "wavefront_mrays": round(wrays / wwall / 1e6, 2),
         "frame_hbm_frac_wavefront_formula": round(survey_bytes / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "nodes_per_ray": round(n_node, 3), "tris_per_ray": round(n_tri, 3), "traversal_occupancy": occupancy,
