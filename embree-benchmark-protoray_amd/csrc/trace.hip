@@ -144,16 +144,18 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     // Frame lookahead (kModeDiffuse, LookState).  This frame's virtual items,
     // from LookState::counter: [0, F) are its items S..entries-1 (F =
     // entries - S), [F, fresh_end) resume the R paths the previous call
-    // suspended (they are part way through, so they go last).  Once a wave has seen
-    // them all handed out and holds no path of this frame, it takes items of
-    // the next frame from LookState::spec_counter; waves that hold paths of
-    // this frame take nothing more (they stay sparse and issue first).
-    // Every wave starts registered in LookState::live and leaves
-    // it once, when it has seen every item of this frame handed out and holds
-    // no path of this frame; the last to leave writes this call's tag into
-    // each block's stop flag.  After the stop no more items are taken and
-    // next-frame paths leave for the queue: at a bounce, or in the middle of
-    // a ray, which the next call then traces again from its origin.
+    // suspended (they are part way through, so they go last).
+    //  - A wave that has seen all of them handed out takes items of the next
+    //    frame from LookState::spec_counter, but only while it holds no path
+    //    of this frame: waves that do take nothing more, stay sparse and
+    //    issue first (s_setprio).
+    //  - Every wave starts registered in LookState::live and leaves it once,
+    //    at that same point (all handed out, none held); the last to leave
+    //    writes this call's tag into each block's stop flag.
+    //  - After the stop no more items are taken, and next-frame paths leave
+    //    for the queue: at a bounce, or in the middle of a ray, which the
+    //    next call then traces again from its origin.
+    //  - The cooperative tail is off under the lookahead.
     constexpr bool kLookMode = kMode == kModeDiffuse;
     LookState* const look = kLookMode ? pl.look : nullptr;
     uint32_t lk_F = 0, lk_S = 0, lk_fresh_end = 0, lk_target = 0;
