@@ -127,6 +127,21 @@ def test_lookahead_dropped_when_inputs_change(dev, orc, scenes, spheres):
     _same(_run(dev, steps, True), _run(dev, steps, False))
 
 
+@pytest.mark.parametrize("w,h,depth", [(13, 7, 8), (64, 1, 0), (257, 161, 1), (33, 65, 14)])
+def test_lookahead_chain_edge_sizes(dev, orc, scenes, spheres, w, h, depth):
+    """Frames smaller than one wave, one pixel high, with partial 8x8 tiles,
+    depth 0 (paths end at their first hit) and the deepest depth: 5 calls in
+    a chain equal the lookahead-off calls."""
+    cam = _cam(orc, scenes, "spheres_1080p", w, h)
+
+    def setup(d):
+        d.set_camera(cam)
+        d.init_frame(w, h)
+
+    steps = [setup] + [lambda d: d.render("diffuseHip", max_depth=depth, albedo=0.7)] * 5
+    _same(_run(dev, steps, True), _run(dev, steps, False))
+
+
 def test_lookahead_dropped_after_scene_rebuild(dev, orc, scenes):
     """A rebuilt scene between two calls of consecutive passes: the second call
     drops the lookahead work of the first (different geometry)."""
