@@ -227,6 +227,7 @@ int prt_hip_destroy(prt_hip_ctx* ctx)
     ctx->scene.release();
     prt_hip_batch_reset(ctx);
     free_ptr(ctx->d_albedo_own);
+    free_ptr(ctx->d_rgb_planes);
     void* ptrs[] = {ctx->d_spill,
                     ctx->d_accum, ctx->d_bgr8, ctx->d_rays[0], ctx->d_rays[1], ctx->d_hits,
                     ctx->d_pix[0], ctx->d_pix[1], ctx->d_rng[0], ctx->d_rng[1], ctx->d_counters,
@@ -1169,13 +1170,32 @@ int prt_hip_render_albedo_image(prt_hip_ctx* ctx, int pass0, int spp, int max_de
     if (albedo_ready(ctx, spp, max_depth, "render_albedo_image")) return 1;
     if (!d_image) return set_error(ctx, "render_albedo_image: null image");
     const int64_t pixels = (int64_t)ctx->width * ctx->height;
+    // The samples run in one launch per chunk of passes (work items (sample,
+    // path), so the chunk pays one tail): each path leaves its RGB in its
+    // sample's plane, and the planes are added to the accumulator in pass
+    // order, which keeps the sums of one launch per pass.  Planes take at most
+    // kRgbPlanesMaxBytes.
+    constexpr int64_t kRgbPlanesMaxBytes = (int64_t)1 << 30;
+    const int64_t plane_bytes = (int64_t)sizeof(float4) * pixels;
+    const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(spp, kRgbPlanesMaxBytes / plane_bytes));
+    if (ctx->rgb_planes_cap < (int64_t)chunk * pixels) {
+        free_ptr(ctx->d_rgb_planes);
+        ctx->d_rgb_planes = nullptr;
+        ctx->rgb_planes_cap = 0;
+        PRT_TRY(ctx, hipMalloc(&ctx->d_rgb_planes, (size_t)chunk * plane_bytes));
+        ctx->rgb_planes_cap = (int64_t)chunk * pixels;
+    }
     PRT_TRY(ctx, hipMemsetAsync(ctx->d_accum, 0, sizeof(float4) * pixels, ctx->stream));
     PRT_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     PRT_TRY(ctx, launch_frame_begin(ctx, (uint32_t)frame_entries(ctx)));
-    for (int s = 0; s < spp; ++s) {
-        PathLaunch p = frame_path_launch(ctx, pass0 + s, max_depth);
+    for (int s0 = 0; s0 < spp; s0 += chunk) {
+        const int n = std::min(chunk, spp - s0);
+        PathLaunch p = frame_path_launch(ctx, pass0 + s0, max_depth);
+        p.spp = n;
+        p.rgb_planes = ctx->d_rgb_planes;
         PRT_TRY(ctx, launch_zero_u32(ctx->d_counters + 16, 1, ctx->stream));
         PRT_TRY(ctx, launch_paths_rgb(ctx, p, (ctx->profiling & PRT_PROFILE_COUNTERS) != 0));
+        PRT_TRY(ctx, launch_rgb_planes_add(ctx, ctx->d_rgb_planes, n));
     }
     PRT_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     PRT_TRY(ctx, launch_accum_mean_rgb(ctx, d_image));

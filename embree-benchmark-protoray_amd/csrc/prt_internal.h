@@ -219,6 +219,8 @@ struct prt_hip_ctx {
     // per-triangle RGB albedo: owned copy or an external device pointer
     float* d_albedo_own = nullptr;
     const float* d_albedo = nullptr;
+    float4* d_rgb_planes = nullptr;  // per-sample RGB planes of the albedo forward
+    int64_t rgb_planes_cap = 0;      // ... in float4
     hipEvent_t kev[4 * 16] = {};    // per-launch timing events (trace/shade x begin/end)
 
     // cooperative tail of the fused diffuse frame (prt_hip_set_tail)
@@ -302,6 +304,10 @@ struct PathLaunch {
     float inv_spp;
     const float* dldi;               // dLoss/dImage, 3 floats per pixel
     float* grad;                     // dLoss/dAlbedo, 3 floats per triangle (accumulated)
+    // RGB forward: when set, work items are (sample, path) for passes pass ..
+    // pass + spp - 1, and each path writes its RGB to plane sample (W*H
+    // float4 each) instead of adding to the accumulator
+    float4* rgb_planes;
     // the "Fast" renderers: frame of the geometric normal whatever attributes
     // the mesh has, and each pass overwrites the accumulator
     bool fast;
@@ -343,6 +349,8 @@ hipError_t launch_shade_primary(prt_hip_ctx* ctx, bool fast);
 hipError_t launch_tonemap(prt_hip_ctx* ctx);
 hipError_t launch_batch_begin(prt_hip_ctx* ctx);
 hipError_t launch_accum_mean_rgb(prt_hip_ctx* ctx, float* image);
+// adds planes 0..spp-1 (W*H float4 each) to the accumulator, in that order
+hipError_t launch_rgb_planes_add(prt_hip_ctx* ctx, const float4* planes, int spp);
 hipError_t launch_batch_finalize(prt_hip_ctx* ctx);
 hipError_t launch_batch_advance(prt_hip_ctx* ctx);
 hipError_t launch_unpack_rays(const float4* rays, const int32_t* pix, const uint32_t* rng,

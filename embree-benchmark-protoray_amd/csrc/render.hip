@@ -372,6 +372,26 @@ static __global__ __launch_bounds__(kBlock) void k_accum_mean_rgb(const float4* 
     image[3 * i + 2] = a.z * inv;
 }
 
+static __global__ __launch_bounds__(kBlock) void k_rgb_planes_add(const float4* planes, int spp, int n,
+                                                                 float4* accum)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float4 a = accum[i];
+    for (int s = 0; s < spp; ++s) {
+        const float4 c = planes[(size_t)s * n + i];
+        a = make_float4(a.x + c.x, a.y + c.y, a.z + c.z, a.w + 1.0f);
+    }
+    accum[i] = a;
+}
+
+hipError_t launch_rgb_planes_add(prt_hip_ctx* ctx, const float4* planes, int spp)
+{
+    const int n = ctx->width * ctx->height;
+    k_rgb_planes_add<<<(n + kBlock - 1) / kBlock, kBlock, 0, ctx->stream>>>(planes, spp, n, ctx->d_accum);
+    return hipGetLastError();
+}
+
 hipError_t launch_accum_mean_rgb(prt_hip_ctx* ctx, float* image)
 {
     const int n = ctx->width * ctx->height;

@@ -77,7 +77,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     const uint64_t lane_lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint32_t count = kRayQuery ? (tl.count_ptr ? *tl.count_ptr : tl.count)
                          : (kMode == kModeBatch ? bl.total_work
-                                                : (kMode == kModeAdjoint ? pl.entries * (uint32_t)pl.spp : pl.entries));
+                                                : (kMode == kModeAdjoint || (kMode == kModeRgb && pl.rgb_planes)
+                                                       ? pl.entries * (uint32_t)pl.spp
+                                                       : pl.entries));
     uint2* my_spill = spill + ((size_t)blockIdx.x * kBlock + tid) * (size_t)stack_entries;
     const int spill_cap = stack_entries;
     if constexpr (kCount && !kRayQuery && kMode != kModeBatch) {
@@ -370,8 +372,14 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                     *sample_out = c;
                 } else if (kMode == kModeRgb) {
                     const float cg = best_prim < 0 ? lg : 0.0f, cb = best_prim < 0 ? lb : 0.0f;
-                    const float4 a = *acc;
-                    *acc = make_float4(a.x + c, a.y + cg, a.z + cb, a.w + 1.0f);
+                    if (pl.rgb_planes) {
+                        // one plane per sample; they are added in pass order afterwards
+                        pl.rgb_planes[(size_t)(path_pass - pl.pass) * (size_t)(pl.width * pl.height) + pixel] =
+                            make_float4(c, cg, cb, 0.0f);
+                    } else {
+                        const float4 a = *acc;
+                        *acc = make_float4(a.x + c, a.y + cg, a.z + cb, a.w + 1.0f);
+                    }
                 } else if (kLookMode && spec) {
                     pl.side[pixel] = make_uint2(__float_as_uint(c), pl.look_gen);
                 } else {
@@ -548,7 +556,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                             path = spec ? my : lk_S + my;
                             path_pass = pl.pass + (spec ? 1 : 0);
                         }
-                        if (kMode == kModeAdjoint) {
+                        if (kMode == kModeAdjoint || (kMode == kModeRgb && pl.rgb_planes)) {
                             const uint32_t sample = my / pl.entries;
                             path = my - sample * pl.entries;
                             path_pass = pl.pass + (int)sample;

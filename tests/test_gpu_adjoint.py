@@ -3,6 +3,7 @@
 Bars (tolerances written here):
   - forward image vs the oracle's double-precision evaluation of the same
     paths: 2e-6 relative per pixel (fp32 products of <= 7 albedos);
+  - the one-launch forward vs one render per pass: bit-identical;
   - adjoint vs the oracle's exact double adjoint: 1e-4 relative on the
     gradient norm and on every entry above 1e-3 of the largest;
   - adjoint vs central finite differences (h = 1e-3) of the L2 loss against a
@@ -47,6 +48,24 @@ def test_forward_and_adjoint_match_oracle(dev, orc, scenes):
     assert np.all(np.abs(grad[big] - gref[big]) <= 1e-4 * np.abs(gref[big]) + 1e-7)
     # the adjoint traces every path once and every escaped path a second time
     assert tape.rays <= ast["ray"] <= 2 * tape.rays
+
+
+def test_forward_one_launch_equals_pass_by_pass(dev, orc, scenes):
+    """The forward renders its spp in one launch (per-sample planes added in
+    pass order): bit-identical to one spp-1 render per pass, accumulated in
+    pass order and divided by spp in fp32, as one launch per pass would."""
+    w, h, spp, depth = 72, 56, 5, 6
+    mesh, cam, a, b = _setup(dev, orc, scenes, w, h, cells=32, boxes=16)
+    dev.set_albedo(a)
+    img, st = dev.render_albedo_image(pass0=3, spp=spp, max_depth=depth)
+    acc = np.zeros((h, w, 3), np.float32)
+    rays = 0
+    for s in range(spp):
+        one, ost = dev.render_albedo_image(pass0=3 + s, spp=1, max_depth=depth)
+        acc = acc + one  # a spp-1 image is the sample itself (x * 1/1)
+        rays += ost["ray"]
+    assert np.array_equal(img, acc * (np.float32(1.0) / np.float32(spp)))
+    assert st["ray"] == rays
 
 
 def test_adjoint_vs_central_differences_256_triangles(dev, orc, scenes):
