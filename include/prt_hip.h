@@ -263,7 +263,10 @@ int prt_hip_set_albedo_device(prt_hip_ctx* ctx, const float* d_rgb);
 /* Mean RGB image over passes pass0 .. pass0+spp-1 with the per-triangle
  * albedo: a path that escapes after hitting t_0..t_{K-1} carries
  * prod_k albedo(t_k) per channel, one hitting at max_depth carries 0.
- * image: 3 floats per pixel, row-major.  Resets the accumulation buffer. */
+ * image: 3 floats per pixel, row-major.  Resets the accumulation buffer.
+ * The samples run in one launch per chunk, through per-sample RGB planes
+ * that the context keeps (W*H*16 bytes per sample, at most 1 GiB); the
+ * result equals one launch per pass bit for bit. */
 int prt_hip_render_albedo_image(prt_hip_ctx* ctx, int pass0, int spp, int max_depth, float* d_image,
                                 prt_render_stats* out_stats);
 int prt_hip_render_albedo_image_host(prt_hip_ctx* ctx, int pass0, int spp, int max_depth, float* image,
