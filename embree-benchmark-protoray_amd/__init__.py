@@ -369,10 +369,11 @@ class HipDevice:
         self._check(self.lib.prt_hip_set_tail(self.ctx, ids[mode]), "set_tail")
 
     def set_lookahead(self, enable: bool = True) -> None:
-        """Frame lookahead of the fused diffuse renderer (default on): idle
-        lanes at the end of a frame start the next pass's paths, which the next
-        render call keeps if it renders that pass with the same inputs.  Images
-        and ray counts per call do not change."""
+        """Frame lookahead of the fused diffuse renderer (default on): in a
+        call that continues the previous one (next pass, same inputs), idle
+        lanes at the end of the frame start the next pass's paths, which the
+        next call keeps if it continues this one.  Images and ray counts per
+        call do not change."""
         self._check(self.lib.prt_hip_set_lookahead(self.ctx, 1 if enable else 0), "set_lookahead")
 
     def set_pipeline(self, pipeline: str) -> None:

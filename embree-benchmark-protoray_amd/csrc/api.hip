@@ -548,7 +548,7 @@ int ensure_look_resources(prt_hip_ctx* ctx)
         ctx->look_q_cap = cap;
         fresh = true;
     }
-    if (fresh) ctx->look_prev_on = false;
+    if (fresh) ctx->look_prev_spec = false;  // nothing the previous call left is kept
     return 0;
 }
 
@@ -595,10 +595,13 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
     const int tiles_x = (ctx->width + 7) / 8;
     const uint32_t entries = (uint32_t)frame_entries(ctx);
 
+    // Lookahead: only in a call that continues the previous call's inputs
+    // (the next pass, same scene, camera, size, depth and albedo), so a
+    // camera that moves every frame renders each frame exactly as without
+    // it; the work is kept only if that previous call ran the lookahead too.
     const bool look_on = fused && renderer == PRT_RENDERER_DIFFUSE && !fast && !counting && ctx->lookahead != 0;
     prt_hip_ctx::LookKey key{};
     if (look_on) {
-        if (ensure_look_resources(ctx)) return 1;
         key.scene_gen = ctx->scene_gen;
         key.cam = ctx->cam;
         key.width = ctx->width;
@@ -608,8 +611,10 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
         key.grid = ctx->trace_grid;
         std::memcpy(&key.albedo_bits, &albedo, sizeof(float));
     }
-    const bool look_valid = look_on && ctx->look_prev_on && look_follows(ctx->look_prev, key);
+    const bool look_run = look_on && ctx->look_prev_on && look_follows(ctx->look_prev, key);
     ctx->look_prev_on = false;  // set again once this call has succeeded
+    if (look_run && ensure_look_resources(ctx)) return 1;
+    const bool look_valid = look_run && ctx->look_prev_spec;
 
     PRT_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (fused) {
@@ -631,7 +636,7 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
         p.stats = ctx->d_stats;
         p.fast = fast;
         p.coop = ctx->tail_coop;
-        if (look_on) {
+        if (look_run) {
             ctx->look_gen += 1;
             PRT_TRY(ctx, launch_look_begin(ctx, look_valid, entries));
             p.look = ctx->d_look;
@@ -688,6 +693,7 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
     if (look_on) {
         ctx->look_prev = key;
         ctx->look_prev_on = true;
+        ctx->look_prev_spec = look_run;
     }
     ctx->pass += 1;
     if (out) {

@@ -242,7 +242,8 @@ struct prt_hip_ctx {
         int width, height, max_depth, pass, grid;
         uint32_t albedo_bits;
     } look_prev{};
-    bool look_prev_on = false;
+    bool look_prev_on = false;    // look_prev holds the inputs of the last successful diffuse call
+    bool look_prev_spec = false;  // ... and that call ran the lookahead (its state is meaningful)
 
     // BVH8 collapse (prt_hip_set_bvh_collapse)
     int bvh_collapse = PRT_COLLAPSE_SAH;
