@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             if (live && fresh_out && !hold) look_leave();
             if (!live && !stop) {
                 const uint32_t f = __hip_atomic_load(stop_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                stop = __builtin_amdgcn_readfirstlane(f) == pl.look_gen;
+                stop = (uint32_t)__builtin_amdgcn_readfirstlane(f) == pl.look_gen;
                 if (stop) exhausted = true;
             }
             // after the stop only next-frame paths are left: those in the
@@ -851,7 +851,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             // waited for with the next node's)
             if (kLookMode && look && !live && !stop && (++poll_steps & 7u) == 0) {
                 const uint32_t f = __hip_atomic_load(stop_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (__builtin_amdgcn_readfirstlane(f) == pl.look_gen) {
+                if ((uint32_t)__builtin_amdgcn_readfirstlane(f) == pl.look_gen) {
                     stop = true;
                     exhausted = true;
                     break;
