@@ -52,6 +52,23 @@ using namespace prt;
 
 namespace {
 
+// Counters and statistics share one device allocation (64 u32, then 32 u64);
+// a call reads both back with one copy into the context's pinned buffer.
+struct ReadBack {
+    uint32_t counters[64];
+    unsigned long long st[32];
+};
+static_assert(sizeof(ReadBack) == 64 * sizeof(uint32_t) + 32 * sizeof(unsigned long long), "read-back layout");
+
+int read_back(prt_hip_ctx* ctx, const ReadBack** out)
+{
+    PRT_TRY(ctx, hipMemcpyAsync(ctx->h_readback, ctx->d_counters, sizeof(ReadBack), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    *out = static_cast<const ReadBack*>(ctx->h_readback);
+    return 0;
+}
+
 int use_device(prt_hip_ctx* ctx)
 {
     if (!ctx) return 1;
@@ -203,7 +220,10 @@ int prt_hip_create(int device_index, prt_hip_ctx** out_ctx)
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
         // counters ([48] batch work counter) and statistics share one
         // allocation, so a frame reads both back with one copy
-        hipMalloc(&ctx->d_counters, 64 * sizeof(uint32_t) + 32 * sizeof(unsigned long long)) != hipSuccess) {
+        hipMalloc(&ctx->d_counters, 64 * sizeof(uint32_t) + 32 * sizeof(unsigned long long)) != hipSuccess ||
+        // pinned, so the per-call read-back is one direct copy
+        hipHostMalloc(&ctx->h_readback, 64 * sizeof(uint32_t) + 32 * sizeof(unsigned long long),
+                      hipHostMallocDefault) != hipSuccess) {
         prt_hip_destroy(ctx);
         return 5;
     }
@@ -238,6 +258,7 @@ int prt_hip_destroy(prt_hip_ctx* ctx)
         if (e) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->h_readback) (void)hipHostFree(ctx->h_readback);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -676,15 +697,10 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
         }
     }
     PRT_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    struct {
-        uint32_t counters[64];
-        unsigned long long st[32];
-    } rb;
-    static_assert(sizeof(rb) == 64 * sizeof(uint32_t) + 32 * sizeof(unsigned long long), "read-back layout");
-    PRT_TRY(ctx, hipMemcpyAsync(&rb, ctx->d_counters, sizeof(rb), hipMemcpyDeviceToHost, ctx->stream));
-    const uint32_t* counters = rb.counters;
-    const unsigned long long* st = rb.st;
-    PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const ReadBack* rb = nullptr;
+    if (read_back(ctx, &rb)) return 1;
+    const uint32_t* counters = rb->counters;
+    const unsigned long long* st = rb->st;
     if (counters[32])
         return set_error(ctx, (counters[32] & 2u) ? "render: lookahead queue overflow"
                                                   : "render: traversal stack overflow");
@@ -961,12 +977,10 @@ int prt_hip_batch_render(prt_hip_ctx* ctx, prt_batch_stats* out)
         return 1;
     }
     PRT_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    unsigned long long st[8];
-    uint32_t err = 0;
-    PRT_TRY(ctx, hipMemcpyAsync(st, ctx->d_stats, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
-    PRT_TRY(ctx, hipMemcpyAsync(&err, ctx->d_counters + 32, 4, hipMemcpyDeviceToHost, ctx->stream));
-    PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    if (err) return set_error(ctx, "batch_render: traversal stack overflow");
+    const ReadBack* rb = nullptr;
+    if (read_back(ctx, &rb)) return 1;
+    const unsigned long long* st = rb->st;
+    if (rb->counters[32]) return set_error(ctx, "batch_render: traversal stack overflow");
     float ms = 0.0f;
     PRT_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     int64_t paths = 0;
@@ -1117,12 +1131,10 @@ int albedo_ready(prt_hip_ctx* ctx, int spp, int max_depth, const char* what)
 
 int read_path_stats(prt_hip_ctx* ctx, prt_render_stats* out, int64_t paths)
 {
-    unsigned long long st[16];
-    uint32_t err = 0;
-    PRT_TRY(ctx, hipMemcpyAsync(st, ctx->d_stats, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
-    PRT_TRY(ctx, hipMemcpyAsync(&err, ctx->d_counters + 32, 4, hipMemcpyDeviceToHost, ctx->stream));
-    PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    if (err) return set_error(ctx, "traversal stack overflow");
+    const ReadBack* rb = nullptr;
+    if (read_back(ctx, &rb)) return 1;
+    const unsigned long long* st = rb->st;
+    if (rb->counters[32]) return set_error(ctx, "traversal stack overflow");
     float ms = 0.0f;
     PRT_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     if (out) {

@@ -205,6 +205,7 @@ struct prt_hip_ctx {
     int32_t* d_pix[2] = {nullptr, nullptr};
     uint32_t* d_rng[2] = {nullptr, nullptr};
     uint32_t* d_counters = nullptr;           // [0..15] queue sizes, [16..31] fetch, [32] error
+    void* h_readback = nullptr;               // pinned host copy of d_counters + d_stats
     unsigned long long* d_stats = nullptr;    // instrumented counters
 
     // host-array intersect staging
