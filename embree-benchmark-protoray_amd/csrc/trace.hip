@@ -55,7 +55,10 @@ __device__ __forceinline__ float byte_f(uint32_t w, int k)
 
 __device__ __forceinline__ float as_f(uint32_t u) { return __uint_as_float(u); }
 
-template <bool kCount, int kMode, int kBlock = kTraceBlock, int kStack = kStackLds>
+// kLook: the fused diffuse frame with the frame lookahead compiled in (and the
+// cooperative tail, which it never uses, left out); the plain variant is the
+// other way round, so each carries only the code it runs.
+template <bool kCount, int kMode, int kBlock = kTraceBlock, int kStack = kStackLds, bool kLook = false>
 __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl, PathLaunch pl, BatchLaunch bl,
                                                   uint2* spill, int stack_entries, int refill, uint32_t* err)
 {
@@ -139,8 +142,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     // Tail of a diffuse frame (kModeDiffuse): once out of work with at most
     // 8 paths left, the wave traces them cooperatively, 8 lanes per path
     // (see the cooperative phase after the main loop)
-    constexpr bool kCoop = kMode == kModeDiffuse;
-    const bool coop_on = kCoop && pl.coop != 0 && pl.look == nullptr;
+    constexpr bool kCoop = kMode == kModeDiffuse && !kLook;
+    const bool coop_on = kCoop && pl.coop != 0;
     bool cont = false;  // the path's next ray waits in (o, d) for the cooperative phase
 
     // Frame lookahead (kModeDiffuse, LookState).  This frame's virtual items,
@@ -158,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     //    for the queue: at a bounce, or in the middle of a ray, which the
     //    next call then traces again from its origin.
     //  - The cooperative tail is off under the lookahead.
-    constexpr bool kLookMode = kMode == kModeDiffuse;
+    constexpr bool kLookMode = kMode == kModeDiffuse && kLook;
     LookState* const look = kLookMode ? pl.look : nullptr;
     uint32_t lk_F = 0, lk_S = 0, lk_fresh_end = 0, lk_target = 0;
     if (kLookMode && look) {
@@ -1185,12 +1188,20 @@ hipError_t launch_mode(prt_hip_ctx* ctx, const TraceLaunch& t, const PathLaunch&
     const SceneDevice sc = ctx->scene.device();
     uint32_t* err = ctx->d_counters + 32;
     const BatchLaunch bl{};
-    if (instrumented)
+    if (instrumented) {
         k_trace<true, kMode><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(
             sc, t, p, bl, ctx->d_spill, ctx->stack_entries, kRefill, err);
-    else
-        k_trace<false, kMode><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(
-            sc, t, p, bl, ctx->d_spill, ctx->stack_entries, kRefill, err);
+        return hipGetLastError();
+    }
+    if constexpr (kMode == kModeDiffuse) {
+        if (p.look != nullptr) {
+            k_trace<false, kMode, kTraceBlock, kStackLds, true><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(
+                sc, t, p, bl, ctx->d_spill, ctx->stack_entries, kRefill, err);
+            return hipGetLastError();
+        }
+    }
+    k_trace<false, kMode><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(
+        sc, t, p, bl, ctx->d_spill, ctx->stack_entries, kRefill, err);
     return hipGetLastError();
 }
 
