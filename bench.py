@@ -285,8 +285,8 @@ def main() -> int:
                     help="short run for rocprofv3 --pmc: the fused frames and a calibration copy, no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is "
-              "described in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path is built for one GPU "
+              "(image tiles or whole scenes would shard across GPUs, which is not built)", file=sys.stderr)
         return 2
 
     sys.path.insert(0, ROOT)
