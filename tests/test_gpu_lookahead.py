@@ -82,10 +82,10 @@ def test_lookahead_chain_bit_exact_1080p(dev, orc, scenes, spheres):
 
 
 def test_lookahead_dropped_when_inputs_change(dev, orc, scenes, spheres):
-    """Calls that change the camera, the pass, max depth, albedo, the renderer
-    or the profiling mode in between: each call still equals its lookahead-off
-    counterpart (work started for a pass that is not rendered next, or with
-    other inputs, is dropped)."""
+    """Calls that change the camera, the pass, max depth, albedo, the renderer,
+    the profiling mode or the pipeline in between: each call still equals its
+    lookahead-off counterpart (work started for a pass that is not rendered
+    next, or with other inputs, is dropped)."""
     w, h = 320, 200
     cam_a = _cam(orc, scenes, "spheres_1080p", w, h)
     cam_b = _cam(orc, scenes, "spheres_1080p", w, h, shift=0.5)
@@ -121,9 +121,19 @@ def test_lookahead_dropped_when_inputs_change(dev, orc, scenes, spheres):
     def clear(d):
         d.clear_frame()
 
+    def pipeline(name):
+        def f(d):
+            d.set_pipeline(name)
+        return f
+
+    def diffuse2(d):
+        d.render("diffuse2Hip", max_depth=4, albedo=0.7)
+
     steps = [setup, diffuse(), diffuse(), cam(cam_b), diffuse(), diffuse(), set_pass(7), diffuse(),
              diffuse(depth=5), diffuse(depth=5), diffuse(albedo=0.5), primary, diffuse(), fast, diffuse(),
-             counters(True), diffuse(), counters(False), diffuse(), diffuse(), clear, diffuse(), diffuse()]
+             counters(True), diffuse(), counters(False), diffuse(), diffuse(), clear, diffuse(), diffuse(),
+             diffuse2, diffuse(), diffuse(), pipeline("wavefront"), diffuse(), pipeline("fused"), diffuse(),
+             diffuse(), diffuse()]
     _same(_run(dev, steps, True), _run(dev, steps, False))
 
 
