@@ -630,7 +630,7 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
         key.height = ctx->height;
         key.max_depth = last;
         key.pass = ctx->pass;
-        key.grid = ctx->trace_grid;
+        key.grid = ctx->diffuse_grid;
         std::memcpy(&key.albedo_bits, &albedo, sizeof(float));
     }
     const bool look_run = look_on && ctx->look_prev_on && look_follows(ctx->look_prev, key);

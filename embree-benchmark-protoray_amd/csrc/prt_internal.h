@@ -32,6 +32,7 @@ static_assert(sizeof(Bvh8Node) == 80, "BVH8 node must be 80 bytes");
 
 constexpr int kLeafMax = 3;           // triangles per leaf slot
 constexpr int kTraceBlock = 256;      // threads per trace workgroup
+constexpr int64_t kLargeSceneBytes = int64_t(256) << 20;  // nodes + triangles beyond MI355X's 256 MiB Infinity Cache
 constexpr int kMaxQueueDepth = 16;    // max_depth + 1 <= 16
 
 // Triangle record: the three fp32 vertices (36 B), in BVH8 leaf order.
@@ -188,6 +189,8 @@ struct prt_hip_ctx {
     uint2* d_spill = nullptr;
     int64_t spill_bytes = 0;
     int trace_grid = 0;        // persistent workgroups per trace launch
+    int diffuse_grid = 0;      // persistent workgroups per diffuse render launch (for the current scene)
+    int trace_per_cu = 0;      // occupancy limit of the persistent trace kernel, workgroups per CU
     int stack_entries = 0;     // max stack depth per lane (LDS + spill)
 
     // frame

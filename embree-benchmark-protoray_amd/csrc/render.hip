@@ -352,7 +352,7 @@ hipError_t launch_frame_begin(prt_hip_ctx* ctx, uint32_t entries)
 hipError_t launch_look_begin(prt_hip_ctx* ctx, bool valid, uint32_t entries)
 {
     k_look_begin<<<1, 64, 0, ctx->stream>>>(ctx->d_look, entries, valid ? 1 : 0, ctx->look_q_cap,
-                                            (uint32_t)ctx->trace_grid * (uint32_t)(kTraceBlock / 64), ctx->d_stats);
+                                            (uint32_t)ctx->diffuse_grid * (uint32_t)(kTraceBlock / 64), ctx->d_stats);
     if (valid) {
         const int n = ctx->width * ctx->height;
         k_look_merge<<<(n + kBlock - 1) / kBlock, kBlock, 0, ctx->stream>>>(ctx->d_look_side, ctx->d_accum, n,

@@ -1188,19 +1188,20 @@ hipError_t launch_mode(prt_hip_ctx* ctx, const TraceLaunch& t, const PathLaunch&
     const SceneDevice sc = ctx->scene.device();
     uint32_t* err = ctx->d_counters + 32;
     const BatchLaunch bl{};
+    const int grid = kMode == kModeDiffuse ? ctx->diffuse_grid : ctx->trace_grid;
     if (instrumented) {
-        k_trace<true, kMode><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(
+        k_trace<true, kMode><<<grid, kTraceBlock, 0, ctx->stream>>>(
             sc, t, p, bl, ctx->d_spill, ctx->stack_entries, kRefill, err);
         return hipGetLastError();
     }
     if constexpr (kMode == kModeDiffuse) {
         if (p.look != nullptr) {
-            k_trace<false, kMode, kTraceBlock, kStackLds, true><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(
+            k_trace<false, kMode, kTraceBlock, kStackLds, true><<<grid, kTraceBlock, 0, ctx->stream>>>(
                 sc, t, p, bl, ctx->d_spill, ctx->stack_entries, kRefill, err);
             return hipGetLastError();
         }
     }
-    k_trace<false, kMode><<<ctx->trace_grid, kTraceBlock, 0, ctx->stream>>>(
+    k_trace<false, kMode><<<grid, kTraceBlock, 0, ctx->stream>>>(
         sc, t, p, bl, ctx->d_spill, ctx->stack_entries, kRefill, err);
     return hipGetLastError();
 }
@@ -1235,14 +1236,25 @@ hipError_t launch_batch_variant(prt_hip_ctx* ctx, const BatchLaunch& bl, int blo
 
 int ensure_trace_resources(prt_hip_ctx* ctx)
 {
-    if (ctx->trace_grid == 0) {
+    if (ctx->trace_per_cu == 0) {
         int per_cu = 0;
         PRT_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, kModeDiffuse>,
                                                                   kTraceBlock, 0));
         if (per_cu * kTraceBlock > 2048) per_cu = 2048 / kTraceBlock;
         if (per_cu < 1) per_cu = 1;
-        ctx->trace_grid = per_cu * ctx->num_cus;
+        ctx->trace_per_cu = per_cu;
     }
+    ctx->trace_grid = ctx->trace_per_cu * ctx->num_cus;
+    // Diffuse render launches on a scene whose nodes and triangles outgrow the
+    // memory-side cache (256 MiB) run one workgroup per CU fewer: fewer paths
+    // in flight evict less of it (the albedo and trace launches keep the full
+    // occupancy, which is faster for them)
+    const SceneBuffers& s = ctx->scene;
+    const int64_t scene_bytes =
+        s.node_count * (int64_t)sizeof(Bvh8Node) + s.tri_count * (int64_t)(9 * sizeof(float) + sizeof(int32_t));
+    int per_cu = ctx->trace_per_cu;
+    if (scene_bytes > kLargeSceneBytes && per_cu > 1) per_cu -= 1;
+    ctx->diffuse_grid = per_cu * ctx->num_cus;
     // worst case stack: 7 pushes per level of the tree, plus slack; sized for
     // the deepest scene and for every lane a CU can hold (2048), so any
     // persistent launch shape fits
