@@ -1246,9 +1246,11 @@ int ensure_trace_resources(prt_hip_ctx* ctx)
     }
     ctx->trace_grid = ctx->trace_per_cu * ctx->num_cus;
     // Diffuse render launches on a scene whose nodes and triangles outgrow the
-    // memory-side cache (256 MiB) run one workgroup per CU fewer: fewer paths
-    // in flight evict less of it (the albedo and trace launches keep the full
-    // occupancy, which is faster for them)
+    // memory-side cache (256 MiB) run one workgroup per CU fewer.  Paths on such
+    // a scene take long, so a frame's tail (its last paths, in sparse waves)
+    // is long, and fewer resident waves shorten it by more than they slow the
+    // rest; the albedo and trace launches keep the full occupancy, which is
+    // faster for them
     const SceneBuffers& s = ctx->scene;
     const int64_t scene_bytes =
         s.node_count * (int64_t)sizeof(Bvh8Node) + s.tri_count * (int64_t)(9 * sizeof(float) + sizeof(int32_t));
