@@ -7,7 +7,8 @@
   triangle once with its own vertices; 200,000 random rays give bit-identical
   closest-hit records to the oracle (overlapping boxes produce coincident
   faces, so this exercises the (t, primId) tie rule at scale); a 960x540
-  frame at depth 8 is bit-identical to the oracle's.
+  frame at depth 8 is bit-identical to the oracle's; a 1080p chain with the
+  frame lookahead equals the chain without it.
 """
 import numpy as np
 import pytest
@@ -92,3 +93,31 @@ def test_city_intersect_and_frame_bit_exact(dev, orc, scenes, city):
     r = sc.render(cam, w, h, spp=1, max_depth=cfg["max_depth"], albedo=cfg["albedo"], renderer=1, numerics=0)
     assert st["ray"] == r["rays"]
     assert np.array_equal(dev.accum(), r["accum"])
+
+
+def test_city_lookahead_chain_bit_exact(dev, orc, scenes, city):
+    """The city's nodes and triangles exceed 256 MiB, so its diffuse launches
+    run one workgroup per CU fewer than the spheres': a chain of 4 calls at
+    1920x1080 with the frame lookahead equals the same calls without it, call
+    by call (accumulator and ray count)."""
+    mesh, _ = city
+    dev.upload_scene(mesh.idx, mesh.pos)
+    dev.build()
+    cfg = scenes.CONFIGS["city_4k"]
+    w, h = 1920, 1080
+    cam = _cam(orc, scenes, "city_4k", w, h)
+    out = {}
+    try:
+        for look in (True, False):
+            dev.set_lookahead(look)
+            dev.set_camera(cam)
+            dev.init_frame(w, h)
+            out[look] = []
+            for _ in range(4):
+                st = dev.render("diffuseHip", max_depth=cfg["max_depth"], albedo=cfg["albedo"])
+                out[look].append((dev.accum(), st["ray"]))
+    finally:
+        dev.set_lookahead(True)
+    for k, ((a, ra), (b, rb)) in enumerate(zip(out[True], out[False])):
+        assert ra == rb, f"call {k}: ray count {ra} with lookahead, {rb} without"
+        assert np.array_equal(a, b), f"call {k}: {int((a != b).any(-1).sum())} pixels differ"
