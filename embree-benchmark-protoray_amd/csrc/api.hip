@@ -11,6 +11,7 @@
 #include <cstddef>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "prt_internal.h"
@@ -129,6 +130,20 @@ int ensure_queues(prt_hip_ctx* ctx, int64_t entries)
 int64_t frame_entries(const prt_hip_ctx* ctx)
 {
     return (int64_t)((ctx->width + 7) / 8) * ((ctx->height + 7) / 8) * 64;
+}
+
+// Adds the trace and shade times of the wavefront launches whose events sit
+// in the first n of the 16 per-depth event sets.
+int add_wave_timing(prt_hip_ctx* ctx, int n, double& trace_ms, double& shade_ms)
+{
+    for (int k = 0; k < n; ++k) {
+        float a = 0.0f, b = 0.0f;
+        PRT_TRY(ctx, hipEventElapsedTime(&a, ctx->kev[4 * k + 0], ctx->kev[4 * k + 1]));
+        PRT_TRY(ctx, hipEventElapsedTime(&b, ctx->kev[4 * k + 1], ctx->kev[4 * k + 2]));
+        trace_ms += a;
+        shade_ms += b;
+    }
+    return 0;
 }
 
 int check_trace_error(prt_hip_ctx* ctx)
@@ -531,6 +546,8 @@ int prt_hip_set_lookahead(prt_hip_ctx* ctx, int enable)
     return 0;
 }
 
+}  // extern "C"
+
 namespace {
 
 // Buffers of the frame lookahead: the state, one side entry per pixel and two
@@ -584,6 +601,8 @@ bool look_follows(const prt_hip_ctx::LookKey& prev, const prt_hip_ctx::LookKey& 
 
 }  // namespace
 
+extern "C" {
+
 int prt_hip_set_pipeline(prt_hip_ctx* ctx, int pipeline)
 {
     if (!ctx) return 1;
@@ -605,14 +624,17 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
         return set_error(ctx, "render: invalid renderer type");
     if (renderer == PRT_RENDERER_DIFFUSE2 && ctx->pipeline != PRT_PIPELINE_FUSED)
         return set_error(ctx, "render: diffuse2 runs on the fused pipeline only");
-    if (max_depth < 0 || max_depth + 1 >= kMaxQueueDepth)
-        return set_error(ctx, "render: max_depth must be in [0, 14]");
+    if (max_depth < 0 || max_depth > kMaxDepth)
+        return set_error(ctx, "render: max_depth must be in [0, " + std::to_string(kMaxDepth) + "]");
     if (ensure_trace_resources(ctx)) return 1;
 
     const bool timing = (ctx->profiling & PRT_PROFILE_TIMING) != 0;
     const bool counting = (ctx->profiling & PRT_PROFILE_COUNTERS) != 0;
     const bool fused = ctx->pipeline == PRT_PIPELINE_FUSED;
     const int last = renderer == PRT_RENDERER_PRIMARY ? 0 : max_depth;
+    int wave_last = last;  // wavefront: the last depth whose queue counter is meaningful
+    double wave_trace_ms = 0.0, wave_shade_ms = 0.0;  // wavefront timing of the depths already read
+    int pending_timed = 0;  // ... and the number of depths whose events are still to be read
     const int tiles_x = (ctx->width + 7) / 8;
     const uint32_t entries = (uint32_t)frame_entries(ctx);
 
@@ -677,25 +699,52 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
     } else {
         PRT_TRY(ctx, launch_raygen(ctx, ctx->pass, ctx->d_rays[0], ctx->d_pix[0], ctx->d_rng[0]));
         float throughput = 1.0f;
+        int traced = -1;  // deepest depth launched
+        bool stopped = false;  // left the loop early: no path left
         for (int depth = 0; depth <= last; ++depth) {
+            traced = depth;
+            // queue size and fetch counter of depth k live in slot k mod 16
+            const int slot = depth & (kMaxQueueDepth - 1);
             TraceLaunch t{};
             t.rays = ctx->d_rays[depth & 1];
             t.hits = ctx->d_hits;
-            t.count_ptr = ctx->d_counters + depth;
+            t.count_ptr = ctx->d_counters + slot;
             t.count = 0;
-            t.fetch_ptr = ctx->d_counters + 16 + depth;
+            t.fetch_ptr = ctx->d_counters + 16 + slot;
             t.stats = counting ? ctx->d_stats : nullptr;
-            if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[4 * depth + 0], ctx->stream));
+            const int ev = slot;  // timing events of depth k: set k mod 16, read every 16 depths
+            if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[4 * ev + 0], ctx->stream));
             PRT_TRY(ctx, launch_trace(ctx, t, counting));
-            if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[4 * depth + 1], ctx->stream));
+            if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[4 * ev + 1], ctx->stream));
             if (renderer == PRT_RENDERER_PRIMARY) {
                 PRT_TRY(ctx, launch_shade_primary(ctx, fast));
             } else {
+                // the next depth's slot was last used by depth + 1 - 16: move
+                // its count to the statistics and clear it first
+                if (depth + 1 >= kMaxQueueDepth && depth < last)
+                    PRT_TRY(ctx, launch_queue_recycle(ctx, depth + 1 - kMaxQueueDepth));
                 PRT_TRY(ctx, launch_shade_diffuse(ctx, depth, max_depth, throughput, fast));
                 throughput *= albedo;
             }
-            if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[4 * depth + 2], ctx->stream));
+            if (timing) PRT_TRY(ctx, hipEventRecord(ctx->kev[4 * ev + 2], ctx->stream));
+            // deep frames: stop once no path is left (as the reference's
+            // while (count > 0)), checked every 16 depths, where the timing
+            // events of those 16 depths are read before they are reused
+            if (renderer != PRT_RENDERER_PRIMARY && depth < last && slot == kMaxQueueDepth - 1) {
+                uint32_t left = 0;
+                PRT_TRY(ctx, hipMemcpyAsync(&left, ctx->d_counters + ((depth + 1) & (kMaxQueueDepth - 1)), 4,
+                                            hipMemcpyDeviceToHost, ctx->stream));
+                PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
+                if (timing && add_wave_timing(ctx, kMaxQueueDepth, wave_trace_ms, wave_shade_ms)) return 1;
+                if (left == 0) {
+                    wave_last = depth + 1;  // the remaining depths trace nothing
+                    stopped = true;
+                    break;
+                }
+            }
         }
+        // events not yet read: those since the last 16-depth check (none after a stop)
+        if (timing && !stopped) pending_timed = (traced & (kMaxQueueDepth - 1)) + 1;
     }
     PRT_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     const ReadBack* rb = nullptr;
@@ -717,12 +766,22 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
         std::memset(out, 0, sizeof(*out));
         const int64_t live = (int64_t)ctx->width * ctx->height;
         int64_t rays = 0;
-        for (int k = 0; k <= last; ++k) {
-            const int64_t q = fused ? (int64_t)st[16 + k] : (k == 0 ? live : (int64_t)counters[k]);
-            out->queue[k] = q;
-            rays += q;
+        if (fused) {
+            // per-depth bins filled by the instrumented kernel (last bin: depths >= 15)
+            for (int k = 0; k < kMaxQueueDepth; ++k) out->queue[k] = (int64_t)st[16 + k];
+            rays = (int64_t)st[3];
+        } else {
+            // depth 0: every pixel; depths whose ring slot was reused later
+            // were moved to the statistics bins by the recycle launches
+            const int end = std::min(wave_last, last);
+            for (int k = 0; k <= end; ++k) {
+                const bool recycled = k + kMaxQueueDepth <= end;
+                const int64_t q = k == 0 ? live : (recycled ? 0 : (int64_t)counters[k & (kMaxQueueDepth - 1)]);
+                out->queue[std::min(k, kMaxQueueDepth - 1)] += q;
+            }
+            for (int b = 1; b < kMaxQueueDepth; ++b) out->queue[b] += (int64_t)st[16 + b];
+            for (int b = 0; b < kMaxQueueDepth; ++b) rays += out->queue[b];
         }
-        if (fused) rays = (int64_t)st[3];
         out->rays = rays;
         out->paths = live;
         out->ms = ms;
@@ -733,13 +792,9 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
                 PRT_TRY(ctx, hipEventElapsedTime(&a, ctx->kev[0], ctx->kev[1]));
                 out->trace_ms = a;
             } else {
-                for (int depth = 0; depth <= last; ++depth) {
-                    float a = 0.0f, b = 0.0f;
-                    PRT_TRY(ctx, hipEventElapsedTime(&a, ctx->kev[4 * depth + 0], ctx->kev[4 * depth + 1]));
-                    PRT_TRY(ctx, hipEventElapsedTime(&b, ctx->kev[4 * depth + 1], ctx->kev[4 * depth + 2]));
-                    out->trace_ms += a;
-                    out->shade_ms += b;
-                }
+                if (add_wave_timing(ctx, pending_timed, wave_trace_ms, wave_shade_ms)) return 1;
+                out->trace_ms = wave_trace_ms;
+                out->shade_ms = wave_shade_ms;
             }
         }
         if (counting) {
@@ -756,6 +811,8 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
     }
     return 0;
 }
+
+}  // extern "C"
 
 namespace {
 
@@ -823,6 +880,8 @@ int batch_enqueue(prt_hip_ctx* ctx, const BatchTuning& tu, bool instrumented, bo
 }
 
 }  // namespace
+
+extern "C" {
 
 int prt_hip_batch_scene_add(prt_hip_ctx* ctx, const int32_t* idx3, int64_t tri_count, const float* pos3,
                             int64_t vertex_count, prt_build_stats* out_stats, int32_t* out_scene)
@@ -902,8 +961,9 @@ int prt_hip_batch_set_jobs(prt_hip_ctx* ctx, const prt_batch_job* jobs, int n_jo
             return set_error(ctx, "batch_set_jobs: unknown scene");
         if (in.width <= 0 || in.height <= 0 || (int64_t)in.width * in.height > ((int64_t)1 << 28))
             return set_error(ctx, "batch_set_jobs: bad image size");
-        if (in.spp < 1 || in.max_depth < 0 || in.max_depth + 1 >= kMaxQueueDepth)
-            return set_error(ctx, "batch_set_jobs: spp must be >= 1 and max_depth in [0, 14]");
+        if (in.spp < 1 || in.max_depth < 0 || in.max_depth > kMaxDepth)
+            return set_error(ctx, "batch_set_jobs: spp must be >= 1 and max_depth in [0, " +
+                                      std::to_string(kMaxDepth) + "]");
         if (!(in.camera.lens_radius >= 0.0f) || !(in.camera.focal_distance > 0.0f))
             return set_error(ctx, "batch_set_jobs: lens radius must be >= 0 and focal distance > 0");
         BatchJobDevice& d = dj[j];
@@ -958,6 +1018,9 @@ int prt_hip_batch_render(prt_hip_ctx* ctx, prt_batch_stats* out)
     if (ensure_trace_resources(ctx)) return 1;
     const bool counting = (ctx->profiling & PRT_PROFILE_COUNTERS) != 0;
     const bool graph = ctx->batch_use_graph && !counting;
+    // the graph holds the spill buffer and its per-lane stride as kernel
+    // arguments: a later build of a deeper scene replaces both
+    if (ctx->batch_graph_exec && ctx->batch_graph_spill_gen != ctx->spill_gen) batch_drop_graph(ctx);
     if (graph && !ctx->batch_graph_exec) {
         PRT_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         const int rc = batch_enqueue(ctx, ctx->batch_tuning, false, true);
@@ -970,6 +1033,7 @@ int prt_hip_batch_render(prt_hip_ctx* ctx, prt_batch_stats* out)
         PRT_TRY(ctx, e);
         ctx->batch_graph = g;
         PRT_TRY(ctx, hipGraphInstantiate(&ctx->batch_graph_exec, g, nullptr, nullptr, 0));
+        ctx->batch_graph_spill_gen = ctx->spill_gen;
     }
     PRT_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (graph) {
@@ -1095,6 +1159,8 @@ int prt_hip_batch_set_graph(prt_hip_ctx* ctx, int enable)
     return 0;
 }
 
+}  // extern "C"
+
 namespace {
 
 PathLaunch frame_path_launch(prt_hip_ctx* ctx, int pass, int max_depth)
@@ -1125,8 +1191,9 @@ int albedo_ready(prt_hip_ctx* ctx, int spp, int max_depth, const char* what)
     if (!ctx->scene.bvh_ready) return set_error(ctx, std::string(what) + ": no BVH");
     if (!ctx->cam_ready || !ctx->d_accum) return set_error(ctx, std::string(what) + ": no camera or frame");
     if (!ctx->d_albedo) return set_error(ctx, std::string(what) + ": no per-triangle albedo");
-    if (spp < 1 || max_depth < 0 || max_depth + 1 >= kMaxQueueDepth)
-        return set_error(ctx, std::string(what) + ": spp must be >= 1 and max_depth in [0, 14]");
+    if (spp < 1 || max_depth < 0 || max_depth > kMaxDepth)
+        return set_error(ctx, std::string(what) + ": spp must be >= 1 and max_depth in [0, " +
+                                  std::to_string(kMaxDepth) + "]");
     return ensure_trace_resources(ctx);
 }
 
@@ -1161,6 +1228,8 @@ int read_path_stats(prt_hip_ctx* ctx, prt_render_stats* out, int64_t paths)
 }
 
 }  // namespace
+
+extern "C" {
 
 int prt_hip_set_albedo_host(prt_hip_ctx* ctx, const float* rgb)
 {

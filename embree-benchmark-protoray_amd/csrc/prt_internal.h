@@ -33,7 +33,8 @@ static_assert(sizeof(Bvh8Node) == 80, "BVH8 node must be 80 bytes");
 constexpr int kLeafMax = 3;           // triangles per leaf slot
 constexpr int kTraceBlock = 256;      // threads per trace workgroup
 constexpr int64_t kLargeSceneBytes = int64_t(256) << 20;  // nodes + triangles beyond MI355X's 256 MiB Infinity Cache
-constexpr int kMaxQueueDepth = 16;    // max_depth + 1 <= 16
+constexpr int kMaxQueueDepth = 16;    // per-depth ray statistics bins; the last bin holds depths >= 15
+constexpr int kMaxDepth = 1024;       // largest max_depth accepted (an R^k throughput below 1 is 0 in fp32 long before)
 
 // Triangle record: the three fp32 vertices (36 B), in BVH8 leaf order.
 struct TriRecord {
@@ -183,6 +184,7 @@ struct prt_hip_ctx {
     prt::BatchTuning batch_tuning;
     hipGraph_t batch_graph = nullptr;
     hipGraphExec_t batch_graph_exec = nullptr;
+    uint64_t batch_graph_spill_gen = 0;  // spill_gen the graph was captured with (its kernel arguments)
     bool batch_use_graph = true;
 
     // traversal spill stack (global part of the short stack)
@@ -191,7 +193,8 @@ struct prt_hip_ctx {
     int trace_grid = 0;        // persistent workgroups per trace launch
     int diffuse_grid = 0;      // persistent workgroups per diffuse render launch (for the current scene)
     int trace_per_cu = 0;      // occupancy limit of the persistent trace kernel, workgroups per CU
-    int stack_entries = 0;     // max stack depth per lane (LDS + spill)
+    int stack_entries = 0;     // max stack depth per lane (LDS + spill); only grows
+    uint64_t spill_gen = 0;    // bumped whenever d_spill or stack_entries change
 
     // frame
     int width = 0, height = 0;
@@ -350,6 +353,9 @@ hipError_t launch_frame_begin(prt_hip_ctx* ctx, uint32_t entries);
 Camera to_camera(const prt_camera& c);
 // fast: the "Fast" renderers (geometric-normal frame, accumulator overwritten)
 hipError_t launch_shade_diffuse(prt_hip_ctx* ctx, int depth, int max_depth, float throughput, bool fast);
+// wavefront frames deeper than 16: frees the counter slot of old_depth (its
+// count moves to the per-depth statistics) for depth old_depth + 16
+hipError_t launch_queue_recycle(prt_hip_ctx* ctx, int old_depth);
 hipError_t launch_shade_primary(prt_hip_ctx* ctx, bool fast);
 hipError_t launch_tonemap(prt_hip_ctx* ctx);
 hipError_t launch_batch_begin(prt_hip_ctx* ctx);

@@ -266,6 +266,20 @@ __global__ __launch_bounds__(kBlock) void k_zero_u32(uint32_t* p, int n)
     if (i < n) p[i] = 0u;
 }
 
+// Wavefront frames deeper than 16: the queue-size and fetch-counter slot of
+// depth k is k mod 16.  Before depth d (d >= 16) reuses its slot, the count
+// of depth d - 16 moves to its statistics bin (the last bin holds depths >=
+// 15; depth 0's count is the pixel count, known on the host) and both
+// counters are cleared.
+__global__ void k_queue_recycle(uint32_t* counters, unsigned long long* stats, int old_depth)
+{
+    if (threadIdx.x != 0) return;
+    const int slot = old_depth & (kMaxQueueDepth - 1);
+    if (old_depth > 0) stats[16 + min(old_depth, kMaxQueueDepth - 1)] += counters[slot];
+    counters[slot] = 0u;
+    counters[16 + slot] = 0u;
+}
+
 // ---- batched form
 __global__ __launch_bounds__(64) void k_batch_begin(uint32_t* counters, unsigned long long* stats)
 {
@@ -420,12 +434,19 @@ hipError_t launch_batch_advance(prt_hip_ctx* ctx)
     return hipGetLastError();
 }
 
+hipError_t launch_queue_recycle(prt_hip_ctx* ctx, int old_depth)
+{
+    k_queue_recycle<<<1, 64, 0, ctx->stream>>>(ctx->d_counters, ctx->d_stats, old_depth);
+    return hipGetLastError();
+}
+
 hipError_t launch_shade_diffuse(prt_hip_ctx* ctx, int depth, int max_depth, float throughput, bool fast)
 {
     const int cur = depth & 1;
+    const int slot = depth & (kMaxQueueDepth - 1), next = (depth + 1) & (kMaxQueueDepth - 1);
     k_shade_diffuse<<<persistent_blocks(ctx), kBlock, 0, ctx->stream>>>(
-        ctx->d_rays[cur], ctx->d_hits, ctx->d_pix[cur], ctx->d_rng[cur], ctx->d_counters + depth,
-        ctx->d_rays[1 - cur], ctx->d_pix[1 - cur], ctx->d_rng[1 - cur], ctx->d_counters + depth + 1,
+        ctx->d_rays[cur], ctx->d_hits, ctx->d_pix[cur], ctx->d_rng[cur], ctx->d_counters + slot,
+        ctx->d_rays[1 - cur], ctx->d_pix[1 - cur], ctx->d_rng[1 - cur], ctx->d_counters + next,
         ctx->d_accum, ctx->scene.d_idx, ctx->scene.d_pos, fast ? nullptr : ctx->scene.d_normals,
         fast ? nullptr : ctx->scene.d_texcoords, throughput, depth == max_depth ? 1 : 0, fast);
     return hipGetLastError();

@@ -282,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             }
             if (kMode == kModeDiffuse2 && shadow) {
                 // shadow ray done: light from the sky unless blocked (MIS weight 1/2)
-                if constexpr (kCount) atomicAdd(&depth_hist[depth], 1u);
+                if constexpr (kCount) atomicAdd(&depth_hist[min(depth, kMaxQueueDepth - 1)], 1u);
                 ++n_rays;
                 if (best_prim < 0) lsum += lw * 0.5f;
                 shadow = false;
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                 active = false;
                 break;
             }
-            if constexpr (kCount && kMode != kModeBatch) atomicAdd(&depth_hist[depth], 1u);
+            if constexpr (kCount && kMode != kModeBatch) atomicAdd(&depth_hist[min(depth, kMaxQueueDepth - 1)], 1u);
             if (kLookMode && spec) ++n_spec_rays;
             else ++n_rays;
             float4* acc = kMode == kModeBatch ? nullptr : pl.accum + pixel;
@@ -1106,7 +1106,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                 // its first lane counts the ray and writes the pixel)
                 if (active) {
                     if constexpr (kCount) {
-                        if (gj == 0) atomicAdd(&depth_hist[depth], 1u);
+                        if (gj == 0) atomicAdd(&depth_hist[min(depth, kMaxQueueDepth - 1)], 1u);
                     }
                     if (gj == 0) ++n_rays;
                     float4* acc = pl.accum + pixel;
@@ -1262,12 +1262,20 @@ int ensure_trace_resources(prt_hip_ctx* ctx)
     // persistent launch shape fits
     int depth = ctx->scene.bvh_depth;
     for (const SceneBuffers& b : ctx->batch_scenes) depth = depth > b.bvh_depth ? depth : b.bvh_depth;
+    // The per-lane stride only grows, so a shallower scene keeps the buffer;
+    // a captured launch graph holds the buffer and stride as kernel
+    // arguments, so every change bumps spill_gen, which the graph's owner
+    // compares before a replay.
     int need = 7 * (depth + 1) + 8;
     if (need < kStackLds) need = kStackLds;
+    if (need < ctx->stack_entries) need = ctx->stack_entries;
     const int64_t lanes = (int64_t)ctx->num_cus * 2048;
     const int64_t bytes = lanes * need * (int64_t)sizeof(uint2);
     if (bytes > ctx->spill_bytes || need != ctx->stack_entries) {
+        ctx->spill_gen += 1;
         if (ctx->d_spill) { (void)hipFree(ctx->d_spill); ctx->d_spill = nullptr; }
+        ctx->spill_bytes = 0;
+        ctx->stack_entries = 0;
         PRT_TRY(ctx, hipMalloc(&ctx->d_spill, bytes));
         ctx->spill_bytes = bytes;
         ctx->stack_entries = need;
@@ -1279,6 +1287,7 @@ hipError_t launch_trace(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumente
 {
     return launch_mode<kModeTrace>(ctx, t, PathLaunch{}, instrumented);
 }
+
 
 hipError_t launch_paths_rgb(prt_hip_ctx* ctx, const PathLaunch& p, bool instrumented)
 {

@@ -57,7 +57,11 @@ typedef struct prt_hit {
 } prt_hit;
 
 /* Camera, the data of ThinLensCameraCuda (camera/thin_lens_camera_cuda.h:23-30).
- * Only the pinhole form (lens_radius == 0) is supported. */
+ * lens_radius == 0 is the pinhole camera (camera/pinhole_camera_cuda.cuh:24-28);
+ * lens_radius > 0 is the thin lens (camera/thin_lens_camera_cuda.cuh:25-31):
+ * the origin moves to a uniform disk sample of that radius in the basis_u /
+ * basis_v plane and the ray passes through the point of the pinhole ray at
+ * focal_distance. */
 typedef struct prt_camera {
     float origin[3];
     float image_o[3];
@@ -88,9 +92,10 @@ typedef struct prt_render_stats {
     double ms;
     double mray;
     int64_t paths;        /* live primary rays = W*H                       */
-    int64_t queue[16];    /* rays traced at depth k (k <= max_depth); with
-                             the fused pipeline only when PRT_PROFILE_COUNTERS
-                             is on, else 0                                  */
+    int64_t queue[16];    /* rays traced at depth k (k <= max_depth), the
+                             last entry summing depths >= 15; with the fused
+                             pipeline only when PRT_PROFILE_COUNTERS is on,
+                             else 0                                         */
     /* filled only when prt_hip_set_profiling enables them:                  */
     double trace_ms;      /* summed device time of the closest-hit launches */
     double shade_ms;      /* summed device time of the shading launches     */
@@ -209,7 +214,11 @@ int prt_hip_frame_clear(prt_hip_ctx* ctx);    /* also resets the pass counter */
 /* --- rendering (DiffuseRendererCuda::render / PrimaryRendererCuda::render) */
 /* One sample per pixel per call, pass = number of calls since the last clear
  * (render/diffuse_renderer_cuda.cu:210).  albedo is the grey reflectance R
- * (the reference hard-codes 0.8, render/diffuse_integrator_single.h:43). */
+ * (the reference hard-codes 0.8, render/diffuse_integrator_single.h:43).
+ * max_depth in [0, 1024] (the reference takes any maxDepth prop,
+ * render/device_cuda.cpp:68; a throughput of R^k below 1 is 0 in fp32 long
+ * before depth 1024); the wavefront pipeline stops early once no path is left,
+ * as the reference's while (count > 0). */
 int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo,
                    prt_render_stats* out_stats);
 /* Sets the pass index the next prt_hip_render call uses. */
@@ -299,7 +308,7 @@ typedef struct prt_batch_job {
     int32_t scene;        /* index returned by prt_hip_batch_scene_add          */
     int32_t width, height;
     int32_t spp;          /* samples per pixel per prt_hip_batch_render call    */
-    int32_t max_depth;    /* 0..14                                              */
+    int32_t max_depth;    /* 0..1024                                            */
     int32_t pass0;        /* sampler pass of the first sample                   */
     float albedo;
     prt_camera camera;    /* pinhole                                            */

@@ -108,8 +108,33 @@ def test_batch_rejects_bad_jobs(prt, scenes):
     cam = prt.camera_from_view([0, 0, 3], 0, 0, math.radians(45), 1.0)
     with prt.HipDevice(0) as dev:
         dev.batch_add_scene(mesh.idx, mesh.pos)
-        for bad in ({"scene": 1}, {"spp": 0}, {"max_depth": 15}, {"width": 0}):
+        for bad in ({"scene": 1}, {"spp": 0}, {"max_depth": 1025}, {"max_depth": -1}, {"width": 0}):
             job = {"scene": 0, "width": 8, "height": 8, "spp": 1, "max_depth": 2, "camera": cam}
             job.update(bad)
             with pytest.raises(RuntimeError):
                 dev.batch_set_jobs([job])
+
+
+def test_batch_graph_after_deeper_plain_build(prt, scenes):
+    """A captured batch graph holds the traversal spill buffer and its per-lane
+    stride.  Building a deeper plain scene on the same context afterwards
+    replaces both; the next batch render must replay a graph captured with
+    the new ones and still equal plain stream launches."""
+    meshes, jobs = _jobs(prt, scenes)
+    deep = scenes.scene_spheres(1002, count=64, subdiv=4)
+    out = {}
+    depths = {}
+    for graph in (True, False):
+        with prt.HipDevice(0) as dev:
+            batch_depth = max(dev.batch_add_scene(m.idx, m.pos)[1]["depth"] for m in meshes)
+            dev.batch_set_graph(graph)
+            dev.batch_set_jobs(jobs)
+            dev.batch_render()
+            dev.upload_scene(deep.idx, deep.pos)
+            depths[graph] = (batch_depth, dev.build()["depth"])
+            dev.batch_render()
+            out[graph] = [dev.batch_accum(j).copy() for j in range(len(jobs))]
+    batch_depth, plain_depth = depths[True]
+    assert plain_depth > batch_depth, "the plain scene must need a deeper traversal stack"
+    for j, (x, y) in enumerate(zip(out[True], out[False])):
+        assert np.array_equal(x, y), f"job {j}: graph replay after the rebuild differs"

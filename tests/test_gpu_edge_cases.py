@@ -130,7 +130,7 @@ def test_error_paths_leave_the_context_usable(prt, orc, scenes):
             d.render("diffuseHip")
         d.init_frame(16, 16)
         with pytest.raises(RuntimeError, match="max_depth"):
-            d.render("diffuseHip", max_depth=15)
+            d.render("diffuseHip", max_depth=1025)
         with pytest.raises(RuntimeError, match="invalid renderer"):
             d.lib.prt_hip_render(d.ctx, 7, 4, 0.8, None) == 0 or d._check(1, "render")
         with pytest.raises(ValueError):

@@ -2,8 +2,11 @@
 include/prt_hip.h declares; host-only entry points behave."""
 import ctypes
 import os
+import shutil
+import subprocess
 
 import numpy as np
+import pytest
 
 
 def test_header_declares_the_abi(prt):
@@ -16,6 +19,22 @@ def test_library_exports_every_declared_symbol(prt):
     missing = [n for n in prt.declared_symbols() if not hasattr(lib, n)]
     assert not missing, f"missing exports: {missing}"
     assert lib.prt_hip_abi_version() == 4
+
+
+def test_library_exports_only_the_c_abi(prt):
+    """Helpers, kernels' host stubs and template instantiations stay internal:
+    the dynamic symbol table holds the prt_hip_* entry points (and the HIP
+    toolchain's compilation-unit ids) and nothing else."""
+    if shutil.which("nm") is None:
+        pytest.skip("nm not available")
+    out = subprocess.run(["nm", "-D", "--defined-only", prt.LIB_PATH], check=True, capture_output=True,
+                         text=True).stdout
+    names = [ln.split()[-1] for ln in out.splitlines() if ln.strip()]
+    extra = [n for n in names if not n.startswith("prt_hip_") and not n.startswith("__hip_cuid_")]
+    assert not extra, f"unexpected exports: {extra}"
+    declared = set(prt.declared_symbols())
+    exported = {n for n in names if n.startswith("prt_hip_")}
+    assert exported == declared, f"exported but not declared: {sorted(exported - declared)}"
 
 
 def test_no_context_without_gpu_is_an_error_not_a_crash(prt):
