@@ -124,6 +124,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "prt_hip_bvh_build": ([P, ctypes.POINTER(PrtBuildStats)], ctypes.c_int),
         "prt_hip_intersect_closest": ([P, P, P, i64], ctypes.c_int),
         "prt_hip_intersect_closest_device": ([P, P, P, i64], ctypes.c_int),
+        "prt_hip_intersect_closest_device_noalloc": ([P, P, P, i64], ctypes.c_int),
         "prt_hip_intersect_counters": ([P, P, P, i64, ctypes.POINTER(PrtTraceCounters)], ctypes.c_int),
         "prt_hip_occluded": ([P, P, P, i64], ctypes.c_int),
         "prt_hip_occluded_device": ([P, P, P, i64], ctypes.c_int),

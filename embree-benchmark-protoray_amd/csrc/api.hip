@@ -10,11 +10,20 @@
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "prt_internal.h"
+
+// Fixed scratch of prt_hip_intersect_closest_device_noalloc: a 256-byte
+// header (u32 [0] work counter, [1] error flag), then the global part of each
+// persistent lane's traversal stack.  Every GPU the library runs on holds its
+// own copy until the process ends.  In host code the name is only a host-side
+// stand-in for the symbol: kernels and copies take the address that
+// hipGetSymbolAddress returns for the current device (scratch_address).
+static __device__ __align__(256) unsigned char scratch[PRT_HIP_NOALLOC_SCRATCH_BYTES];
 
 namespace prt {
 
@@ -68,6 +77,29 @@ int read_back(prt_hip_ctx* ctx, const ReadBack** out)
     PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
     *out = static_cast<const ReadBack*>(ctx->h_readback);
     return 0;
+}
+
+constexpr size_t kScratchHeader = 256;
+constexpr int kScratchMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchMaxDevices] = {};
+
+// Device address of `scratch` on device_index, which must be the current
+// device; looked up once per device and kept.
+hipError_t scratch_address(int device_index, void** out)
+{
+    *out = nullptr;
+    if (device_index < 0 || device_index >= kScratchMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device_index]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device_index] = p;
+    }
+    *out = g_scratch_ptr[device_index];
+    return hipSuccess;
 }
 
 int use_device(prt_hip_ctx* ctx)
@@ -242,6 +274,11 @@ int prt_hip_create(int device_index, prt_hip_ctx** out_ctx)
         prt_hip_destroy(ctx);
         return 5;
     }
+    // the no-allocation query's scratch on this device (the device is current)
+    if (scratch_address(device_index, &ctx->d_scratch) != hipSuccess) {
+        prt_hip_destroy(ctx);
+        return 5;
+    }
     ctx->d_stats = reinterpret_cast<unsigned long long*>(ctx->d_counters + 64);
     ctx->num_cus = prop.multiProcessorCount;
     ctx->device_name = prop.name[0] ? std::string(prop.name) : std::string("AMD ") + prop.gcnArchName;
@@ -371,6 +408,41 @@ int prt_hip_intersect_closest(prt_hip_ctx* ctx, const prt_ray* rays, prt_hit* hi
     PRT_TRY(ctx, hipMemcpyAsync(hits, ctx->d_io_hits, sizeof(prt_hit) * n, hipMemcpyDeviceToHost, ctx->stream));
     PRT_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return check_trace_error(ctx);
+}
+
+int prt_hip_intersect_closest_device_noalloc(prt_hip_ctx* ctx, const prt_ray* d_rays, prt_hit* d_hits, int64_t n)
+{
+    if (use_device(ctx)) return 1;
+    if (!ctx->scene.bvh_ready || ctx->trace_per_cu <= 0)
+        return set_error(ctx, "intersect_noalloc: no BVH (call prt_hip_bvh_build)");
+    if (n < 0 || n > (int64_t)UINT32_MAX / 2) return set_error(ctx, "intersect_noalloc: bad ray count");
+    if (n == 0) return 0;
+    if (!d_rays || !d_hits) return set_error(ctx, "intersect_noalloc: null array");
+    if (!ctx->d_scratch) return set_error(ctx, "intersect_noalloc: no scratch on this device");
+    // the same worst-case stack as the allocating query; the launch keeps as
+    // many persistent workgroups as the fixed scratch holds stacks for
+    int need = 7 * (ctx->scene.bvh_depth + 1) + 8;
+    if (need < 16) need = 16;
+    const size_t per_block = (size_t)kTraceBlock * (size_t)need * sizeof(uint2);
+    const size_t blocks_fit = ((size_t)PRT_HIP_NOALLOC_SCRATCH_BYTES - kScratchHeader) / per_block;
+    if (blocks_fit < 1)
+        return set_error(ctx, "intersect_noalloc: a BVH of depth " + std::to_string(ctx->scene.bvh_depth) +
+                                  " needs " + std::to_string(per_block) + " bytes of stack per workgroup, more than the " +
+                                  std::to_string((size_t)PRT_HIP_NOALLOC_SCRATCH_BYTES) + "-byte scratch holds");
+    int grid = ctx->trace_per_cu * ctx->num_cus;
+    if ((size_t)grid > blocks_fit) grid = (int)blocks_fit;
+    uint32_t* hdr = static_cast<uint32_t*>(ctx->d_scratch);
+    PRT_TRY(ctx, launch_zero_u32(hdr, 2, ctx->stream));  // a kernel, not a memset: safe under graph capture
+    TraceLaunch t{};
+    t.rays = reinterpret_cast<const float4*>(d_rays);
+    t.hits = reinterpret_cast<float4*>(d_hits);
+    t.count_ptr = nullptr;
+    t.count = (uint32_t)n;
+    t.fetch_ptr = hdr;
+    t.stats = nullptr;
+    PRT_TRY(ctx, launch_trace_scratch(ctx, t, reinterpret_cast<uint2*>(static_cast<char*>(ctx->d_scratch) + kScratchHeader),
+                                      need, hdr + 1, grid));
+    return 0;
 }
 
 int prt_hip_occluded_device(prt_hip_ctx* ctx, const prt_ray* d_rays, float* d_out_t, int64_t n)

@@ -252,6 +252,10 @@ struct prt_hip_ctx {
     bool look_prev_on = false;    // look_prev holds the inputs of the last successful diffuse call
     bool look_prev_spec = false;  // ... and that call ran the lookahead (its state is meaningful)
 
+    // device address of the library's fixed scratch array on this context's
+    // GPU (prt_hip_intersect_closest_device_noalloc)
+    void* d_scratch = nullptr;
+
     // BVH8 collapse (prt_hip_set_bvh_collapse)
     int bvh_collapse = PRT_COLLAPSE_SAH;
     float bvh_prim_cost = 1.0f;     // triangle test cost relative to a wide-node visit
@@ -333,6 +337,10 @@ struct PathLaunch {
 };
 int ensure_trace_resources(prt_hip_ctx* ctx);
 hipError_t launch_trace(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumented);
+// closest-hit launch over `grid` persistent workgroups whose stack spill
+// (stack_entries per lane) and error flag the caller provides
+hipError_t launch_trace_scratch(prt_hip_ctx* ctx, const TraceLaunch& t, uint2* spill, int stack_entries,
+                                uint32_t* err, int grid);
 hipError_t launch_occluded(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumented);
 // renderer: PRT_RENDERER_PRIMARY, _DIFFUSE or _DIFFUSE2
 hipError_t launch_paths(prt_hip_ctx* ctx, const PathLaunch& p, int renderer, bool instrumented);

@@ -1288,6 +1288,14 @@ hipError_t launch_trace(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumente
     return launch_mode<kModeTrace>(ctx, t, PathLaunch{}, instrumented);
 }
 
+hipError_t launch_trace_scratch(prt_hip_ctx* ctx, const TraceLaunch& t, uint2* spill, int stack_entries,
+                                uint32_t* err, int grid)
+{
+    k_trace<false, kModeTrace><<<grid, kTraceBlock, 0, ctx->stream>>>(ctx->scene.device(), t, PathLaunch{},
+                                                                     BatchLaunch{}, spill, stack_entries, kRefill,
+                                                                     err);
+    return hipGetLastError();
+}
 
 hipError_t launch_paths_rgb(prt_hip_ctx* ctx, const PathLaunch& p, bool instrumented)
 {

@@ -179,6 +179,23 @@ int prt_hip_intersect_closest(prt_hip_ctx* ctx, const prt_ray* rays, prt_hit* hi
 /* Device arrays, on the context's stream; asynchronous. */
 int prt_hip_intersect_closest_device(prt_hip_ctx* ctx, const prt_ray* d_rays, prt_hit* d_hits,
                                      int64_t n);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  The same query and the same hit records as
+ * prt_hip_intersect_closest_device, on the context's stream, asynchronous,
+ * but it allocates nothing: the traversal's scratch (work counter, error
+ * flag, the global part of each lane's stack) is a fixed device array of
+ * PRT_HIP_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it runs
+ * on until the process ends.  The launch keeps as many persistent
+ * workgroups as that array holds worst-case stacks for (BVH depth D needs
+ * 256 * (7 (D + 1) + 8) * 8 bytes per workgroup), and a BVH too deep for one
+ * workgroup is rejected with an error.  n <= 2^31 - 1 rays.  Every call on
+ * one GPU shares that GPU's array, whatever the context: calls that overlap
+ * in time on one GPU (other streams, other threads) race, so the caller
+ * orders them. */
+#define PRT_HIP_NOALLOC_SCRATCH_BYTES (128u << 20)
+int prt_hip_intersect_closest_device_noalloc(prt_hip_ctx* ctx, const prt_ray* d_rays, prt_hit* d_hits,
+                                             int64_t n);
 /* Same query on the instrumented kernel; adds per-ray node/triangle counts. */
 int prt_hip_intersect_counters(prt_hip_ctx* ctx, const prt_ray* rays, prt_hit* hits, int64_t n,
                                prt_trace_counters* out_counters);
