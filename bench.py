@@ -82,18 +82,132 @@ def fused_bytes(rays: int, paths: int, n_node: float, n_tri: float) -> float:
     return rays * (80 * n_node + 36 * n_tri) + paths * 32
 
 
-def cpu_baseline(scenes, mesh, cfg, cam, frames: int) -> dict:
+def host_threads() -> tuple[int, int]:
+    """Threads the CPU baseline runs on, and the host's configured CPU count.
+    The reference's Tasking runs one worker per hardware thread and reports
+    sysconf(_SC_NPROCESSORS_CONF) (sys/tasking.h:33-52, sys/sysinfo.cpp:50-61).
+    Here the workers are the CPUs this process may run on, capped by the
+    OMP_NUM_THREADS share the host gives one GPU's job when it sets one."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        avail = min(avail, int(share))
+    try:
+        conf = int(os.sysconf("SC_NPROCESSORS_CONF"))
+    except (AttributeError, ValueError, OSError):
+        conf = os.cpu_count() or avail
+    return max(1, avail), conf
+
+
+def cpu_stream(mesh, cfg, cam, frames: int, threads: int) -> dict:
+    """The oracle's restatement of the CPU stream renderer
+    (render/diffuse_renderer_stream.h:82-225) on `frames` full frames."""
     sys.path.insert(0, ROOT)
     from oracle import pyoracle as orc
-    threads = max(1, min(16, os.cpu_count() or 1))
     sc = orc.Scene(mesh.idx, mesh.pos)
     w, h = cfg["width"], cfg["height"]
     r = sc.stream_bench(cam, w, h, 0, h, frames, cfg["max_depth"], cfg["albedo"], threads=threads)
+    sc.close()
+    return r
+
+
+def cpu_baseline(scenes, mesh, cfg, cam, frames: int) -> dict:
+    threads, conf = host_threads()
+    r = cpu_stream(mesh, cfg, cam, frames, threads)
+    w, h = cfg["width"], cfg["height"]
     return {"value": round(r["rays"] / r["seconds"] / 1e6, 3), "unit": "Mrays/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host_cpus_configured": conf,
             "sample": f"{frames} full {w}x{h} frames (passes 0-{frames - 1}), max depth {cfg['max_depth']}, "
                       f"restated stream renderer over a scalar binned-SAH BVH2 (not Embree), "
                       f"{r['rays']} rays in {r['seconds']:.2f} s"}
+
+
+def bench_config1(pkg, scenes, args) -> dict:
+    """The reference's CPU-runnable case: 64x64-cell heightfield in an open
+    box (8,202 triangles), 512x512, 4 spp, max depth 4, R = 0.8, on the GPU
+    (plain fused path, the 4 frames timed) and on the CPU stream baseline
+    (the same 4 frames, repeated until about a second of work)."""
+    cfg = scenes.CONFIGS["heightfield_box_512"]
+    w, h, spp, depth, albedo = cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"], cfg["albedo"]
+    mesh = scenes.build_scene("heightfield_box_512")
+    v = cfg["view"]
+    cam = pkg.camera_from_view(v["pos"], v["angle_x"], v["angle_y"], v["fov_y"], w / h)
+    with pkg.HipDevice(0) as dev:
+        dev.upload_scene(mesh.idx, mesh.pos)
+        dev.build()
+        dev.set_camera(cam)
+        dev.init_frame(w, h)
+        for _ in range(2 * spp):  # warm-up: the same passes' chain, then cleared
+            dev.render("diffuseHip", max_depth=depth, albedo=albedo)
+        best = None
+        for _ in range(5):
+            dev.clear_frame()
+            t0 = time.perf_counter()
+            rays = sum(dev.render("diffuseHip", max_depth=depth, albedo=albedo)["ray"] for _ in range(spp))
+            wall = time.perf_counter() - t0
+            best = wall if best is None or wall < best else best
+    out = {"workload": f"heightfield_box_512: {mesh.tri_count} tris, {w}x{h}, {spp} spp, depth {depth}, R {albedo}",
+           "gpu_mrays": round(rays / best / 1e6, 2), "gpu_ms_per_frame": round(best * 1e3 / spp, 4),
+           "rays_per_image": rays}
+    if not args.no_cpu_baseline:
+        threads, conf = host_threads()
+        r = cpu_stream(mesh, cfg, cam, spp, threads)
+        out["cpu_baseline"] = {"value": round(r["rays"] / r["seconds"] / 1e6, 3), "unit": "Mrays/s",
+                               "cores": threads, "kind": "port", "host_cpus_configured": conf,
+                               "sample": f"{spp} full {w}x{h} frames, {r['rays']} rays in {r['seconds']:.2f} s"}
+        out["gpu_over_cpu"] = round(out["gpu_mrays"] / out["cpu_baseline"]["value"], 1)
+    log(f"config 1 (heightfield_box_512): GPU {out['gpu_mrays']} Mrays/s"
+        + (f", CPU {out['cpu_baseline']['value']} Mrays/s on {out['cpu_baseline']['cores']} threads"
+           if "cpu_baseline" in out else ""))
+    return out
+
+
+def profile_traffic(name: str) -> dict | None:
+    """HBM-side bytes per launch of a kernel and workload from its committed
+    rocprofv3 --pmc summary (profiles/<name>.md), or None."""
+    try:
+        text = open(os.path.join(ROOT, "profiles", name + ".md")).read()
+    except OSError:
+        return None
+    m = re.search(r'\{"traffic_bytes_per_launch": \d+[^}]*\}', text)
+    return json.loads(m.group(0)) if m else None
+
+
+def roofline_entry(kernel: str, algo_bytes: float, kernel_ms: float, rays: int, traffic: dict | None,
+                   source: str) -> dict:
+    achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic["traffic_bytes_per_launch"] if traffic else None,
+            "traffic_source": f"profiles/{source}.md" if traffic else None,
+            "l2_hit_rate": traffic.get("l2_hit_rate") if traffic else None,
+            "valu_busy": traffic.get("valu_busy") if traffic else None,
+            "valu_lane_utilization": traffic.get("valu_lane_utilization") if traffic else None,
+            "kernel": kernel, "bytes_per_ray": round(algo_bytes / max(1, rays), 1),
+            "avg_launch_ms": round(kernel_ms, 4)}
+
+
+def batch_roofline(dev, timed: list, kernel: str, source: str) -> dict:
+    """Roofline of the batched path kernel: per launch rays * (80 * nodes/ray
+    + 36 * tris/ray) of BVH8 node and triangle-record fetches plus 4 B per
+    path (its sample-plane write), with nodes/tris per ray from one
+    instrumented render of the same jobs (the next passes), over the device
+    time of the replayed graph (HIP events around it; the graph's finalize
+    and counter launches are inside, so this is a lower bound)."""
+    dev.set_profiling(counters=True)
+    c = dev.batch_render()
+    dev.set_profiling()
+    n_node = c["nodes"] / max(1, c["ray"])
+    n_tri = c["tris"] / max(1, c["ray"])
+    rays = timed[-1]["ray"]
+    ms = min(x["renderMs"] for x in timed)
+    algo = rays * (80 * n_node + 36 * n_tri) + 4 * timed[-1]["paths"]
+    out = roofline_entry(kernel, algo, ms, rays, profile_traffic(source), source)
+    out.update({"nodes_per_ray": round(n_node, 3), "tris_per_ray": round(n_tri, 3)})
+    return out
 
 
 def bench_batched(pkg, scenes, dev, mesh, cam, cfg, args, plain_mrays: float, wavefront_mrays: float,
@@ -119,6 +233,8 @@ def bench_batched(pkg, scenes, dev, mesh, cam, cfg, args, plain_mrays: float, wa
     wall = time.perf_counter() - t0
     rays = sum(x["ray"] for x in st)
     c2 = rays / wall / 1e6
+    roof2 = batch_roofline(dev, st, "k_trace<batch> (persistent batched path kernel), spheres_1080p job",
+                           "k_trace_batch_spheres_1080p")
     log(f"batched spheres_1080p ({args.steps} spp per launch): {c2:.1f} Mrays/s "
         f"({c2 / plain_mrays:.2f}x plain fused, {c2 / wavefront_mrays:.2f}x plain wavefront), tuning {chosen2}")
 
@@ -143,6 +259,8 @@ def bench_batched(pkg, scenes, dev, mesh, cam, cfg, args, plain_mrays: float, wa
     wall3 = time.perf_counter() - t0
     rays3 = sum(x["ray"] for x in st3)
     c3 = rays3 / wall3 / 1e6
+    roof3 = batch_roofline(dev, st3, "k_trace<batch> (persistent batched path kernel), 32-scene workload",
+                           "k_trace_batch_scenes32")
     log(f"batched {len(specs)} scenes ({tris} triangles, {st3[0]['paths']} paths per launch): {c3:.1f} Mrays/s, "
         f"tuning {chosen3}")
     dev.batch_reset()
@@ -153,7 +271,7 @@ def bench_batched(pkg, scenes, dev, mesh, cam, cfg, args, plain_mrays: float, wa
             "spheres_1080p_tuning": chosen2,
             "scenes32_mrays": round(c3, 2), "scenes32_triangles": tris, "scenes32_paths": st3[0]["paths"],
             "scenes32_rays_per_launch": st3[0]["ray"], "scenes32_tuning": chosen3,
-            "autotune_candidates": len(tune3)}
+            "autotune_candidates": len(tune3), "roofline": roof2, "scenes32_roofline": roof3}
 
 
 def bench_adjoint(pkg, scenes, args) -> dict:
@@ -183,14 +301,27 @@ def bench_adjoint(pkg, scenes, args) -> dict:
         reps = 5
         fwd = [dev.render_albedo_image(0, spp, depth)[1] for _ in range(reps)]
         adj = [dev.adjoint_albedo(0, spp, depth, dldi)[1] for _ in range(reps)]
+        dev.set_profiling(counters=True)
+        cadj = dev.adjoint_albedo(0, spp, depth, dldi)[1]
+        dev.set_profiling()
     f_ms = min(s["renderMs"] for s in fwd)
     a_ms = min(s["renderMs"] for s in adj)
     a_rays = adj[0]["ray"]
+    # adjoint kernel: both traces' node and triangle fetches, one dL/dI read
+    # (12 B) per path and at most one 12 B gradient atomic per replayed ray
+    n_node = cadj["nodes"] / max(1, cadj["ray"])
+    n_tri = cadj["tris"] / max(1, cadj["ray"])
+    replay_rays = a_rays - fwd[0]["ray"]
+    a_bytes = a_rays * (80 * n_node + 36 * n_tri) + 12 * replay_rays + 12 * adj[0]["paths"]
+    a_roof = roofline_entry("k_trace<adjoint> (path-replay albedo adjoint, one launch over all samples)", a_bytes,
+                            a_ms, a_rays, profile_traffic("k_trace_adjoint_heightfield_boxes_512"),
+                            "k_trace_adjoint_heightfield_boxes_512")
+    a_roof.update({"nodes_per_ray": round(n_node, 3), "tris_per_ray": round(n_tri, 3), "replay_rays": replay_rays})
     out = {"workload": "heightfield_boxes_512: 33,536 tris, 512x512, 8 spp, depth 6, per-triangle RGB albedo",
            "forward_ms": round(f_ms, 3), "forward_mrays": round(fwd[0]["ray"] / f_ms / 1e3, 2),
            "adjoint_ms": round(a_ms, 3), "adjoint_mrays": round(a_rays / a_ms / 1e3, 2),
            "adjoint_over_forward": round(a_ms / f_ms, 3), "adjoint_rays": a_rays, "forward_rays": fwd[0]["ray"],
-           "loss": float(((img.astype(np.float64) - target) ** 2).sum())}
+           "loss": float(((img.astype(np.float64) - target) ** 2).sum()), "roofline": a_roof}
     log(f"adjoint: forward {f_ms:.2f} ms, adjoint {a_ms:.2f} ms ({a_ms / f_ms:.2f}x), "
         f"{out['adjoint_mrays']} Mrays/s")
     return out
@@ -279,7 +410,7 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-city", action="store_true")
     ap.add_argument("--city-spp", type=int, default=64)
-    ap.add_argument("--city-adjoint-spp", type=int, default=8)
+    ap.add_argument("--city-adjoint-spp", type=int, default=64)
     ap.add_argument("--batch-scenes", type=int, default=32)
     ap.add_argument("--pmc-pass", action="store_true",
                     help="short run for rocprofv3 --pmc: the fused frames and a calibration copy, no JSON line")
@@ -422,6 +553,7 @@ def main() -> int:
         "cpu_baseline": None,
     }
     dev.close()
+    out["config1"] = bench_config1(pkg, scenes, args)
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(pkg, scenes, args)
     if not args.no_city:

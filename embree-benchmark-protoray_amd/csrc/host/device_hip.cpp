@@ -165,12 +165,14 @@ Props DeviceHip::queryRay(const Ray& r)
         ng = ng3(ng);
         n = ng3(n);
     }
-    Vec3f uv{hit.u, hit.v, 0.0f};
+    // ctx.uv is a Vec2f: the interpolated texture coordinates, or the hit's
+    // (u, v) when the mesh has none (geometry/triangle_mesh.cpp:115,150)
+    Vec2f uv{hit.u, hit.v};
     if (!texcoords_.empty()) {
         const float* a = &texcoords_[2 * (size_t)t[0]];
         const float* b = &texcoords_[2 * (size_t)t[1]];
         const float* c = &texcoords_[2 * (size_t)t[2]];
-        uv = Vec3f{b0 * a[0] + b1 * b[0] + b2 * c[0], b0 * a[1] + b1 * b[1] + b2 * c[1], 0.0f};
+        uv = Vec2f{b0 * a[0] + b1 * b[0] + b2 * c[0], b0 * a[1] + b1 * b[1] + b2 * c[1]};
     }
     const Vec3f p{r.org.x + hit.t * r.dir.x, r.org.y + hit.t * r.dir.y, r.org.z + hit.t * r.dir.z};
     const float eps = std::max(hit.t, std::max(std::fabs(p.x), std::max(std::fabs(p.y), std::fabs(p.z)))) *
@@ -178,7 +180,7 @@ Props DeviceHip::queryRay(const Ray& r)
     const int mat = hit.prim_id < (int32_t)material_ids_.size() ? material_ids_[hit.prim_id] : 0;
     res.set("mat", mat >= 0 && mat < (int)material_names_.size() ? material_names_[mat] : std::string());
     res.set("matId", mat);
-    res.set("primId", (int64_t)hit.prim_id);
+    res.set("primId", (int)hit.prim_id);
     res.set("dist", hit.t);
     res.set("p", p);
     res.set("Ng", Vec3f{ng.x, ng.y, ng.z});
@@ -188,18 +190,38 @@ Props DeviceHip::queryRay(const Ray& r)
     return res;
 }
 
-Props DeviceHip::queryPixel(int x, int y)
+Ray DeviceHip::cameraRay(float ix, float iy) const
 {
-    if (size_.x <= 0 || x < 0 || y < 0 || x >= size_.x || y >= size_.y) return Props();
-    const float ix = ((float)x + 0.5f) / (float)size_.x, iy = ((float)y + 0.5f) / (float)size_.y;
+    // Camera::getRay with the lens sample 0 (render/renderer.h:59-71):
+    // PinholeCamera::getRay (camera/pinhole_camera.h:50-54), or for a lens
+    // radius > 0 ThinLensCamera::getRay (camera/thin_lens_camera.h:38-44),
+    // whose disk sample of 0 is the lens centre, so the ray starts at the
+    // origin and passes through the focal-plane point of the image sample
     float d[3];
     for (int k = 0; k < 3; ++k) d[k] = camera_.image_o[k] + ix * camera_.image_dx[k] + iy * camera_.image_dy[k];
+    if (camera_.lens_radius != 0.0f)
+        for (int k = 0; k < 3; ++k)
+            d[k] = (camera_.origin[k] + camera_.focal_distance * d[k]) - camera_.origin[k];
     const float inv = 1.0f / std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     Ray r;
     r.org = Vec3f{camera_.origin[0], camera_.origin[1], camera_.origin[2]};
     r.dir = Vec3f{d[0] * inv, d[1] * inv, d[2] * inv};
+    return r;
+}
+
+Props DeviceHip::queryPixel(int x, int y)
+{
+    // Renderer::queryPixel (render/renderer.h:57-81): the ray through the
+    // pixel centre; depth is the hit distance along the view direction (the
+    // ray through the image centre)
+    if (size_.x <= 0 || x < 0 || y < 0 || x >= size_.x || y >= size_.y) return Props();
+    const Ray centre = cameraRay(0.5f, 0.5f);
+    const Ray r = cameraRay(((float)x + 0.5f) / (float)size_.x, ((float)y + 0.5f) / (float)size_.y);
     Props res = queryRay(r);
-    if (!res.empty()) res.set("depth", res.get<float>("dist"));
+    if (!res.empty()) {
+        const float cosv = r.dir.x * centre.dir.x + r.dir.y * centre.dir.y + r.dir.z * centre.dir.z;
+        res.set("depth", res.get<float>("dist") * cosv);
+    }
     return res;
 }
 

@@ -36,6 +36,7 @@ public:
 
 private:
     void check(int rc, const char* what);
+    Ray cameraRay(float ix, float iy) const;  // camera ray of image point (ix, iy), lens sample 0
 
     prt_hip_ctx* ctx_ = nullptr;
     Box3f bounds_;

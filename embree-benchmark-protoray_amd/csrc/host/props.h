@@ -12,6 +12,7 @@
 namespace prt {
 
 struct Vec2i { int x = 0, y = 0; };
+struct Vec2f { float x = 0, y = 0; };
 struct Vec3f { float x = 0, y = 0, z = 0; };
 struct Box3f { Vec3f low, high; };
 struct Basis3f { Vec3f U, V, N; };
@@ -30,6 +31,7 @@ public:
     Value(double v) : nums_{v} {}
     Value(bool v) : nums_{v ? 1.0 : 0.0} {}
     Value(const Vec2i& v) : nums_{(double)v.x, (double)v.y} {}
+    Value(const Vec2f& v) : nums_{v.x, v.y} {}
     Value(const Vec3f& v) : nums_{v.x, v.y, v.z} {}
     Value(const Basis3f& b) : nums_{b.U.x, b.U.y, b.U.z, b.V.x, b.V.y, b.V.z, b.N.x, b.N.y, b.N.z} {}
 

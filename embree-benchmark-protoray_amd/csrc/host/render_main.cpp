@@ -138,7 +138,113 @@ void write_ppm(const std::string& path, const std::vector<int32_t>& px, int w, i
     fclose(f);
 }
 
+// The camera of view st.view_id: <scene>.view if present, else the default
+// view (RenderWindow::resetView, main/render/render_window.cpp:400-426),
+// through RenderWindow::makeCamera (main/render/render_window.cpp:307-325).
+void init_view(DeviceHip& device, const Settings& st)
+{
+    const std::string base = st.scene.substr(0, st.scene.find_last_of('.'));
+    View views[kViewSetSize];
+    const Box3f b = device.getSceneBounds();
+    const Vec3f c{0.5f * (b.low.x + b.high.x), 0.5f * (b.low.y + b.high.y), 0.5f * (b.low.z + b.high.z)};
+    const float ext = std::fmax(std::fmax(b.high.x - b.low.x, b.high.y - b.low.y), b.high.z - b.low.z);
+    View def;
+    def.pos = Vec3f{c.x, c.y, 1.7f * ext + c.z};
+    def.fov_y = 45.0f * (3.14159265358979323846f / 180.0f);
+    def.focus = std::sqrt((c.z - def.pos.z) * (c.z - def.pos.z));
+    for (View& v : views) v = def;
+    load_view_set(base + ".view", views);
+    const View& view = views[st.view_id];
+
+    float basis[9];
+    prt_hip_view_basis(view.angle_x, view.angle_y, basis);
+    Props cam;
+    cam.set("type", view.radius == 0.0f ? "pinhole" : "thinlens");
+    cam.set("position", view.pos);
+    cam.set("basis", Basis3f{{basis[0], basis[1], basis[2]}, {basis[3], basis[4], basis[5]},
+                             {basis[6], basis[7], basis[8]}});
+    cam.set("fov", view.fov_y);
+    cam.set("aspectRatio", (float)st.size.x / (float)st.size.y);
+    cam.set("lensRadius", view.radius);
+    cam.set("focalDistance", view.focus);
+    device.initCamera(cam);
+}
+
+void print_query(int i, const Props& res)
+{
+    std::cout << "query " << i << std::endl;
+    for (const auto& kv : res.items()) std::cout << kv.first << "=" << kv.second.text() << std::endl;
+    std::cout << "end" << std::endl;
+}
+
 }  // namespace
+
+// `protoray query <scene.mesh> [-size W,H] [-view N] [-pixel x,y]... [-ray ox,oy,oz,dx,dy,dz]...`
+// Device::queryPixel / Device::queryRay (render/device.h:43-44), which the
+// reference's interactive window calls on a click (RenderWindow, main/render/
+// render_window.cpp:447); here for scripts and tests.  Prints, per query in
+// order, "query <i>", the result's key=value lines (none for a miss), "end".
+int query_main(const std::vector<std::string>& args)
+{
+    Settings st;
+    std::vector<std::pair<bool, std::vector<double>>> queries;  // (is pixel, numbers)
+    std::vector<Option> rest;
+    try {
+        for (const Option& o : parse_options(args)) {
+            if (o.name == "pixel" || o.name == "ray") {
+                std::vector<double> n;
+                std::string v = o.value;
+                for (char& ch : v)
+                    if (ch == ',') ch = ' ';
+                std::istringstream sm(v);
+                double d;
+                while (sm >> d) n.push_back(d);
+                if (n.size() != (o.name == "pixel" ? 2u : 6u)) {
+                    std::cerr << "Error: -" << o.name << " needs " << (o.name == "pixel" ? 2 : 6) << " numbers"
+                              << std::endl;
+                    return 1;
+                }
+                queries.emplace_back(o.name == "pixel", n);
+            } else {
+                rest.push_back(o);
+            }
+        }
+        if (!apply_options(rest, st)) return 1;
+    } catch (const std::exception& e) {
+        std::cerr << "Error: " << e.what() << std::endl;
+        return 1;
+    }
+    if (st.scene.empty()) {
+        std::cerr << "Error: Scene not specified" << std::endl;
+        return 1;
+    }
+    try {
+        DeviceHip device(0);
+        device.initScene(st.scene, st.props);
+        device.initFrame(st.size, st.props);
+        Props rprops = st.props;
+        rprops.set("type", st.renderer);
+        rprops.set("imageSize", st.size);
+        Props build_stats;
+        device.initRenderer(rprops, build_stats);
+        init_view(device, st);
+        for (size_t i = 0; i < queries.size(); ++i) {
+            const std::vector<double>& n = queries[i].second;
+            if (queries[i].first) {
+                print_query((int)i, device.queryPixel((int)n[0], (int)n[1]));
+            } else {
+                Ray r;
+                r.org = Vec3f{(float)n[0], (float)n[1], (float)n[2]};
+                r.dir = Vec3f{(float)n[3], (float)n[4], (float)n[5]};
+                print_query((int)i, device.queryRay(r));
+            }
+        }
+    } catch (const std::exception& e) {
+        std::cerr << "Error: " << e.what() << std::endl;
+        return 1;
+    }
+    return 0;
+}
 
 int render_main(const std::vector<std::string>& args)
 {
@@ -180,32 +286,7 @@ int render_main(const std::vector<std::string>& args)
         device.initRenderer(rprops, build_stats);
         log_line("Build time: " + fmt(build_stats.get<double>("buildMs")) + " ms");
 
-        // view: <scene>.view if present, else the default view (resetView)
-        const std::string base = st.scene.substr(0, st.scene.find_last_of('.'));
-        View views[kViewSetSize];
-        const Box3f b = device.getSceneBounds();
-        const Vec3f c{0.5f * (b.low.x + b.high.x), 0.5f * (b.low.y + b.high.y), 0.5f * (b.low.z + b.high.z)};
-        const float ext = std::fmax(std::fmax(b.high.x - b.low.x, b.high.y - b.low.y), b.high.z - b.low.z);
-        View def;
-        def.pos = Vec3f{c.x, c.y, 1.7f * ext + c.z};
-        def.fov_y = 45.0f * (3.14159265358979323846f / 180.0f);
-        def.focus = std::sqrt((c.z - def.pos.z) * (c.z - def.pos.z));
-        for (View& v : views) v = def;
-        load_view_set(base + ".view", views);
-        const View& view = views[st.view_id];
-
-        float basis[9];
-        prt_hip_view_basis(view.angle_x, view.angle_y, basis);
-        Props cam;
-        cam.set("type", view.radius == 0.0f ? "pinhole" : "thinlens");
-        cam.set("position", view.pos);
-        cam.set("basis", Basis3f{{basis[0], basis[1], basis[2]}, {basis[3], basis[4], basis[5]},
-                                 {basis[6], basis[7], basis[8]}});
-        cam.set("fov", view.fov_y);
-        cam.set("aspectRatio", (float)st.size.x / (float)st.size.y);
-        cam.set("lensRadius", view.radius);
-        cam.set("focalDistance", view.focus);
-        device.initCamera(cam);
+        init_view(device, st);
         device.clearFrame();
 
         const int max_spp = st.props.get("spp", 1);
@@ -342,8 +423,11 @@ int main(int argc, char* argv[])
     std::vector<std::string> args(argv + (argc < 2 ? argc : 2), argv + argc);
     if (cmd == "render") return render_main(args);
     if (cmd == "build") return build_main(args);
+    if (cmd == "query") return query_main(args);
     std::cout << "Usage: protoray render <scene.mesh> [-dev hip] [-r primaryHip|diffuseHip|diffuse2Hip] [-size W,H] "
                  "[-spp N] [-maxDepth D] [-albedo R] [-view N] [-benchmark prefix]" << std::endl
-              << "       protoray build <scene.obj|scene.mesh> [-o out.mesh] [-t mesh]" << std::endl;
+              << "       protoray build <scene.obj|scene.mesh> [-o out.mesh] [-t mesh]" << std::endl
+              << "       protoray query <scene.mesh> [-size W,H] [-view N] [-pixel x,y]... "
+                 "[-ray ox,oy,oz,dx,dy,dz]..." << std::endl;
     return argc < 2 ? 0 : 1;
 }
