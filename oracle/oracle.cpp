@@ -1035,9 +1035,25 @@ void orc_intersect_fp32(void* scp, const float* rays, int64_t n, float* hits, in
 // overwrites the pixel: render/device_cuda.cpp:79-90,
 // render/diffuse_renderer_cuda.cu:109-118);
 // numerics 0 = kernel fp32, 1 = reference fp64.
+int orc_render_rows(void* scp, const float* cam, int w, int h, int row0, int row1, int pass0, int spp,
+                    int max_depth, float albedo, int renderer, int numerics, int threads, float* accum,
+                    float* sumsq, int64_t* rays_out);
+
 int orc_render(void* scp, const float* cam, int w, int h, int pass0, int spp, int max_depth, float albedo,
                int renderer, int numerics, int threads, float* accum, float* sumsq, int64_t* rays_out)
 {
+    return orc_render_rows(scp, cam, w, h, 0, h, pass0, spp, max_depth, albedo, renderer, numerics, threads, accum,
+                           sumsq, rays_out);
+}
+
+// orc_render restricted to image rows [row0, row1) of the w x h frame (same
+// pixel ids, samples and rays as the full frame; other pixels untouched).
+int orc_render_rows(void* scp, const float* cam, int w, int h, int row0, int row1, int pass0, int spp,
+                    int max_depth, float albedo, int renderer, int numerics, int threads, float* accum,
+                    float* sumsq, int64_t* rays_out)
+{
+    row0 = std::max(0, row0);
+    row1 = std::min(h, row1);
     const Scene& sc = *static_cast<Scene*>(scp);
     const bool fast = (renderer & 0x100) != 0;
     renderer &= 0xff;
@@ -1046,11 +1062,11 @@ int orc_render(void* scp, const float* cam, int w, int h, int pass0, int spp, in
     int nt = default_threads(threads);
     std::atomic<int64_t> total_rays{0};
     int rows_per_tile = 8;
-    int ntiles = (h + rows_per_tile - 1) / rows_per_tile;
+    int ntiles = row1 > row0 ? (row1 - row0 + rows_per_tile - 1) / rows_per_tile : 0;
     for (int pass = pass0; pass < pass0 + spp; ++pass) {
         parallel_tiles(ntiles, nt, [&](int tile) {
             int64_t local_rays = 0;
-            for (int y = tile * rows_per_tile; y < std::min(h, (tile + 1) * rows_per_tile); ++y)
+            for (int y = row0 + tile * rows_per_tile; y < std::min(row1, row0 + (tile + 1) * rows_per_tile); ++y)
                 for (int x = 0; x < w; ++x) {
                     int pid = x + y * w;
                     Sampler s{seed_of(pass, pid)};

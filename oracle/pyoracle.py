@@ -46,6 +46,8 @@ def lib() -> ctypes.CDLL:
         L.orc_occluded_fp32.argtypes = [P, P, i64, P, i]
         L.orc_render.argtypes = [P, P, i, i, i, i, i, f, i, i, i, P, P, P]
         L.orc_render.restype = i
+        L.orc_render_rows.argtypes = [P, P, i, i, i, i, i, i, i, f, i, i, i, P, P, P]
+        L.orc_render_rows.restype = i
         L.orc_stream_bench.argtypes = [P, P, i, i, i, i, i, i, i, f, i, P, P, P]
         L.orc_stream_bench.restype = i
         L.orc_bounce.argtypes = [P, i64, i, P]
@@ -188,13 +190,17 @@ class Scene:
         return out
 
     def render(self, cam: np.ndarray, w: int, h: int, spp: int, max_depth: int, albedo: float,
-               renderer: int = 1, numerics: int = 0, pass0: int = 0, threads: int = 0) -> dict:
+               renderer: int = 1, numerics: int = 0, pass0: int = 0, threads: int = 0,
+               rows: tuple[int, int] | None = None) -> dict:
+        """rows=(r0, r1): only image rows r0..r1-1 of the w x h frame (the
+        others stay 0)."""
         cam = camera20(cam)
         accum = np.zeros((h, w, 4), dtype=np.float32)
         sumsq = np.zeros((h, w), dtype=np.float32)
         rays = ctypes.c_int64(0)
-        lib().orc_render(self.h, _p(cam), w, h, pass0, spp, max_depth, albedo, renderer, numerics, threads,
-                         _p(accum), _p(sumsq), ctypes.byref(rays))
+        r0, r1 = rows if rows is not None else (0, h)
+        lib().orc_render_rows(self.h, _p(cam), w, h, r0, r1, pass0, spp, max_depth, albedo, renderer, numerics,
+                              threads, _p(accum), _p(sumsq), ctypes.byref(rays))
         return {"accum": accum, "sumsq": sumsq, "rays": rays.value}
 
     def stream_bench(self, cam: np.ndarray, w: int, h: int, row0: int, row1: int, frames: int,

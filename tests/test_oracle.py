@@ -358,3 +358,18 @@ def test_stream_baseline_renders_the_oracle_image(orc, scenes):
     assert diff.mean() <= 1e-3, f"{int(diff.sum())} of {w * h} pixels differ"
     assert (st["accum"][:, :, 3] == frames).all()
     assert abs(st["rays"] - ref["rays"]) <= 4 * int(diff.sum()) + 0
+
+
+def test_render_rows_equal_full_frame_rows(orc, scenes):
+    """A row band of the single-ray renderer is those rows of the full frame
+    (same pixel ids, samples and rays); the other rows stay empty."""
+    mesh = scenes.scene_heightfield_box(1001, cells=16)
+    sc = orc.Scene(mesh.idx, mesh.pos)
+    v = scenes.CONFIGS["heightfield_box_512"]["view"]
+    w, h = 40, 33
+    cam = orc.camera_from_view(v["pos"], v["angle_x"], v["angle_y"], v["fov_y"], w / h)
+    full = sc.render(cam, w, h, spp=2, max_depth=4, albedo=0.8, pass0=3)
+    band = sc.render(cam, w, h, spp=2, max_depth=4, albedo=0.8, pass0=3, rows=(13, 22))
+    assert np.array_equal(band["accum"][13:22], full["accum"][13:22])
+    assert not band["accum"][:13].any() and not band["accum"][22:].any()
+    assert 0 < band["rays"] < full["rays"]
