@@ -178,10 +178,18 @@ def profile_traffic(name: str) -> dict | None:
 
 def roofline_entry(kernel: str, algo_bytes: float, kernel_ms: float, rays: int, traffic: dict | None,
                    source: str) -> dict:
+    """roofline object of one kernel; `traffic` (the committed --pmc summary)
+    was measured on a launch of `rays_per_launch` rays, so it is scaled to
+    this launch's ray count."""
     achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
+    hbm = None
+    if traffic:
+        hbm = traffic["traffic_bytes_per_launch"]
+        if traffic.get("rays_per_launch"):
+            hbm = int(round(hbm * rays / traffic["rays_per_launch"]))
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic["traffic_bytes_per_launch"] if traffic else None,
+            "traffic": hbm,
             "traffic_source": f"profiles/{source}.md" if traffic else None,
             "l2_hit_rate": traffic.get("l2_hit_rate") if traffic else None,
             "valu_busy": traffic.get("valu_busy") if traffic else None,
@@ -375,7 +383,9 @@ COPY_FLOATS = 1 << 27   # calibration copy: 512 MiB read + 512 MiB written
 
 def pmc_pass(pkg, mesh, cam, cfg, args) -> int:
     """Short run for the rocprofv3 --pmc passes: `--steps` frames of the fused
-    path kernel (the timed kernel), then a plain elementwise copy of
+    path kernel (the timed kernel), 3 batched launches of spheres_1080p x 16
+    spp and 2 of the 32-scene workload, 3 albedo adjoint launches of the
+    differentiable workload, then a plain elementwise copy of
     COPY_FLOATS floats (torch, `dst = src * 1`), whose FETCH_SIZE / WRITE_SIZE
     against its known 512 MiB read and written scale the counters to bytes."""
     import torch
@@ -389,7 +399,49 @@ def pmc_pass(pkg, mesh, cam, cfg, args) -> int:
         dev.render("diffuseHip", max_depth=depth, albedo=albedo)
     rays = sum(dev.render("diffuseHip", max_depth=depth, albedo=albedo)["ray"] for _ in range(args.steps))
     log(f"pmc pass: {args.steps} fused frames, {rays} rays")
+    # the batched kernel: spheres_1080p as one job of 16 spp (graph replays),
+    # then the 32-scene workload, with the launch shapes bench_batched uses
+    dev.batch_add_scene(mesh.idx, mesh.pos)
+    dev.batch_set_jobs([{"scene": 0, "width": w, "height": h, "spp": 16, "max_depth": depth, "pass0": 0,
+                         "albedo": albedo, "camera": cam}])
+    dev.batch_set_tuning(0, 0, 32)
+    for _ in range(3):
+        dev.batch_render()
+    log("pmc pass: 3 batched launches of spheres_1080p x 16 spp")
+    dev.batch_reset()
+    scenes = importlib.import_module(PKG + ".scenes")
+    jobs = []
+    for sp in scenes.batch_workload(args.batch_scenes):
+        m = scenes.scene_mix(sp["seed"], sp["target_tris"])
+        sid, _ = dev.batch_add_scene(m.idx, m.pos)
+        v = sp["view"]
+        jobs.append({"scene": sid, "width": sp["width"], "height": sp["height"], "spp": sp["spp"],
+                     "max_depth": sp["max_depth"], "albedo": sp["albedo"],
+                     "camera": pkg.camera_from_view(v["pos"], v["angle_x"], v["angle_y"], v["fov_y"],
+                                                    sp["width"] / sp["height"])})
+    dev.batch_set_jobs(jobs)
+    dev.batch_set_tuning(2, 0, 32)
+    for _ in range(2):
+        dev.batch_render()
+    log("pmc pass: 2 batched launches of the 32-scene workload")
     dev.close()
+    # the albedo forward and its path-replay adjoint on the differentiable workload
+    acfg = scenes.CONFIGS["heightfield_boxes_512"]
+    amesh = scenes.build_scene("heightfield_boxes_512")
+    av = acfg["view"]
+    aw, ah = acfg["width"], acfg["height"]
+    with pkg.HipDevice(0) as adev:
+        adev.upload_scene(amesh.idx, amesh.pos)
+        adev.build()
+        adev.set_camera(pkg.camera_from_view(av["pos"], av["angle_x"], av["angle_y"], av["fov_y"], aw / ah))
+        adev.init_frame(aw, ah)
+        adev.set_albedo(scenes.albedo_set(amesh.tri_count, acfg["target_seed"]))
+        target, _ = adev.render_albedo_image(0, acfg["spp"], acfg["max_depth"])
+        adev.set_albedo(scenes.albedo_set(amesh.tri_count, acfg["albedo_seed"]))
+        img, _ = adev.render_albedo_image(0, acfg["spp"], acfg["max_depth"])
+        for _ in range(3):
+            adev.adjoint_albedo(0, acfg["spp"], acfg["max_depth"], 2.0 * (img - target))
+    log("pmc pass: 3 adjoint launches of heightfield_boxes_512 (8 spp)")
     src = torch.rand(COPY_FLOATS, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(3):
