@@ -734,7 +734,8 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
 
     PRT_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (fused) {
-        PRT_TRY(ctx, launch_frame_begin(ctx, entries));
+        // a lookahead call clears the counters in its hand-over launch
+        if (!look_run) PRT_TRY(ctx, launch_frame_begin(ctx, entries));
         PathLaunch p{};
         p.cam = to_camera(ctx->cam);
         p.width = ctx->width;
@@ -754,7 +755,7 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
         p.coop = ctx->tail_coop;
         if (look_run) {
             ctx->look_gen += 1;
-            PRT_TRY(ctx, launch_look_begin(ctx, look_valid, entries));
+            PRT_TRY(ctx, launch_look_call_begin(ctx, look_valid, entries));
             p.look = ctx->d_look;
             p.side = ctx->d_look_side;
             p.q_in = ctx->d_look_q[(ctx->look_gen - 1) & 1];

@@ -344,10 +344,11 @@ hipError_t launch_trace_scratch(prt_hip_ctx* ctx, const TraceLaunch& t, uint2* s
 hipError_t launch_occluded(prt_hip_ctx* ctx, const TraceLaunch& t, bool instrumented);
 // renderer: PRT_RENDERER_PRIMARY, _DIFFUSE or _DIFFUSE2
 hipError_t launch_paths(prt_hip_ctx* ctx, const PathLaunch& p, int renderer, bool instrumented);
-// frame lookahead: roll the state to a new call (valid: the previous call
-// rendered the previous pass with the same inputs and lookahead on), then add
-// the side samples of the previous call to the accumulator
-hipError_t launch_look_begin(prt_hip_ctx* ctx, bool valid, uint32_t entries);
+// frame lookahead, in one launch: clear the counters, roll the state to a new
+// call (valid: the previous call rendered the previous pass with the same
+// inputs and lookahead on) and add the side samples of the previous call to
+// the accumulator
+hipError_t launch_look_call_begin(prt_hip_ctx* ctx, bool valid, uint32_t entries);
 hipError_t launch_paths_rgb(prt_hip_ctx* ctx, const PathLaunch& p, bool instrumented);
 hipError_t launch_adjoint(prt_hip_ctx* ctx, const PathLaunch& p, bool instrumented);
 // max_blocks_per_cu != nullptr: only query the variant's occupancy limit

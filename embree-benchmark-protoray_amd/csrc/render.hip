@@ -87,41 +87,52 @@ __device__ __forceinline__ void accum_add(float4* accum, int pixel, float c, boo
     accum[pixel] = make_float4(a.x + c, a.y + c, a.z + c, a.w + 1.0f);
 }
 
-// Frame lookahead: hands the state that the previous call left to this call
-// (LookState).  valid: the previous call rendered the previous pass of the
-// same frame with lookahead on; else its next-frame work is dropped.  The
-// previous call took next-frame items in order from its spec_counter, so the
-// items it started are a prefix of this frame's.
-__global__ void k_look_begin(LookState* ls, uint32_t entries, int valid, uint32_t q_cap, uint32_t waves,
-                             unsigned long long* stats)
+// Set-up of a lookahead call, in one launch before the path kernel.
+//  - Block 0 clears the counters and statistics (as k_frame_begin), then
+//    hands the state that the previous call left to this call (LookState).
+//    valid: the previous call rendered the previous pass of the same frame
+//    with lookahead on; else its next-frame work is dropped.  The previous
+//    call took next-frame items in order from its spec_counter, so the items
+//    it started are a prefix of this frame's.
+//  - With a valid hand-over, every block adds the samples of this frame that
+//    the previous call finished (tagged `tag`) to its pixels' accumulators;
+//    those pixels get no other sample in this call.
+__global__ __launch_bounds__(kBlock) void k_look_call_begin(uint32_t* counters, uint32_t n0,
+                                                            unsigned long long* stats, LookState* ls,
+                                                            uint32_t entries, int valid, uint32_t q_cap,
+                                                            uint32_t waves, const uint2* side, float4* accum, int n,
+                                                            uint32_t tag)
 {
-    if (threadIdx.x != 0) return;
-    uint32_t r = 0, s = 0;
-    unsigned long long carried = 0;
-    if (valid) {
-        s = min(entries, ls->spec_counter);
-        r = min(ls->resume_out, q_cap);  // a call whose queue overflowed failed: no valid successor
-        carried = ls->spec_rays;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < 33) counters[threadIdx.x] = threadIdx.x == 0 ? n0 : 0u;
+        if (threadIdx.x < 32) stats[threadIdx.x] = threadIdx.x == 9 ? ~0ull : 0ull;  // [9]: a minimum
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t r = 0, s = 0;
+            unsigned long long carried = 0;
+            if (valid) {
+                s = min(entries, ls->spec_counter);
+                r = min(ls->resume_out, q_cap);
+                carried = ls->spec_rays;
+            }
+            ls->counter = 0;
+            ls->spec_counter = 0;
+            ls->resume_in = r;
+            ls->started = s;
+            ls->target = r + (entries - s);
+            ls->live = waves;
+            ls->resume_out = 0;
+            ls->spec_rays = 0;
+            stats[3] = carried;
+        }
     }
-    ls->counter = 0;
-    ls->spec_counter = 0;
-    ls->resume_in = r;
-    ls->started = s;
-    ls->target = r + (entries - s);
-    ls->live = waves;
-    ls->resume_out = 0;
-    ls->spec_rays = 0;
-    stats[3] = carried;  // this frame's rays that the previous call traced
-}
-
-// Adds the samples of this frame that the previous call finished (tag) to the
-// accumulator; the pixels they cover get no other sample in this call.
-__global__ __launch_bounds__(kBlock) void k_look_merge(const uint2* side, float4* accum, int n, uint32_t tag)
-{
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint2 e = side[i];
-    if (e.y == tag) accum_add(accum, i, __uint_as_float(e.x), false);
+    if (valid) {
+        const int i = blockIdx.x * kBlock + threadIdx.x;
+        if (i < n) {
+            const uint2 e = side[i];
+            if (e.y == tag) accum_add(accum, i, __uint_as_float(e.x), false);
+        }
+    }
 }
 
 // One bounce of the diffuse integrator for every queued ray: misses add the
@@ -363,15 +374,14 @@ hipError_t launch_frame_begin(prt_hip_ctx* ctx, uint32_t entries)
     return hipGetLastError();
 }
 
-hipError_t launch_look_begin(prt_hip_ctx* ctx, bool valid, uint32_t entries)
+hipError_t launch_look_call_begin(prt_hip_ctx* ctx, bool valid, uint32_t entries)
 {
-    k_look_begin<<<1, 64, 0, ctx->stream>>>(ctx->d_look, entries, valid ? 1 : 0, ctx->look_q_cap,
-                                            (uint32_t)ctx->diffuse_grid * (uint32_t)(kTraceBlock / 64), ctx->d_stats);
-    if (valid) {
-        const int n = ctx->width * ctx->height;
-        k_look_merge<<<(n + kBlock - 1) / kBlock, kBlock, 0, ctx->stream>>>(ctx->d_look_side, ctx->d_accum, n,
-                                                                           ctx->look_gen - 1);
-    }
+    const int n = ctx->width * ctx->height;
+    const int grid = valid ? (n + kBlock - 1) / kBlock : 1;
+    k_look_call_begin<<<grid, kBlock, 0, ctx->stream>>>(
+        ctx->d_counters, entries, ctx->d_stats, ctx->d_look, entries, valid ? 1 : 0, ctx->look_q_cap,
+        (uint32_t)ctx->diffuse_grid * (uint32_t)(kTraceBlock / 64), ctx->d_look_side, ctx->d_accum, n,
+        ctx->look_gen - 1);
     return hipGetLastError();
 }
 
