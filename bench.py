@@ -114,14 +114,22 @@ def cpu_stream(mesh, cfg, cam, frames: int, threads: int) -> dict:
     return r
 
 
+def cpu_tree() -> str:
+    sys.path.insert(0, ROOT)
+    from oracle import pyoracle as orc
+    return ("an 8-wide binned-SAH BVH with AVX2 child tests" if orc.cpu_has_avx2()
+            else "a scalar binned-SAH BVH2")
+
+
 def cpu_baseline(scenes, mesh, cfg, cam, frames: int) -> dict:
     threads, conf = host_threads()
+    tree = cpu_tree()
     r = cpu_stream(mesh, cfg, cam, frames, threads)
     w, h = cfg["width"], cfg["height"]
     return {"value": round(r["rays"] / r["seconds"] / 1e6, 3), "unit": "Mrays/s", "cores": threads,
             "kind": "port", "host_cpus_configured": conf,
             "sample": f"{frames} full {w}x{h} frames (passes 0-{frames - 1}), max depth {cfg['max_depth']}, "
-                      f"restated stream renderer over a scalar binned-SAH BVH2 (not Embree), "
+                      f"restated stream renderer over {tree} (not Embree), "
                       f"{r['rays']} rays in {r['seconds']:.2f} s"}
 
 
@@ -158,7 +166,8 @@ def bench_config1(pkg, scenes, args) -> dict:
         r = cpu_stream(mesh, cfg, cam, frames, threads)
         out["cpu_baseline"] = {"value": round(r["rays"] / r["seconds"] / 1e6, 3), "unit": "Mrays/s",
                                "cores": threads, "kind": "port", "host_cpus_configured": conf,
-                               "sample": f"{frames} full {w}x{h} frames (passes 0-{frames - 1}), {r['rays']} rays "
+                               "sample": f"{frames} full {w}x{h} frames (passes 0-{frames - 1}), restated stream "
+                                         f"renderer over {cpu_tree()} (not Embree), {r['rays']} rays "
                                          f"in {r['seconds']:.2f} s"}
         out["gpu_over_cpu"] = round(out["gpu_mrays"] / out["cpu_baseline"]["value"], 1)
     log(f"config 1 (heightfield_box_512): GPU {out['gpu_mrays']} Mrays/s"
@@ -288,8 +297,8 @@ def bench_adjoint(pkg, scenes, args) -> dict:
     """Differentiable workload: 128x128 heightfield + 64 boxes (33,536
     triangles), 512x512, 8 spp, depth 6, per-triangle RGB albedo U[0.1, 0.9];
     L2 loss against a target rendered from a second albedo set.  Times the
-    forward image (8 one-sample launches) and the path-replay adjoint (one
-    launch over all 8 samples)."""
+    forward image (one launch over all 8 samples) and the path-replay adjoint
+    (one launch over all 8 samples)."""
     cfg = scenes.CONFIGS["heightfield_boxes_512"]
     w, h, spp, depth = cfg["width"], cfg["height"], cfg["spp"], cfg["max_depth"]
     mesh = scenes.build_scene("heightfield_boxes_512")

@@ -34,6 +34,8 @@
 #include <thread>
 #include <vector>
 
+#include <immintrin.h>
+
 namespace {
 
 // ----------------------------------------------------------------- sampler
@@ -141,6 +143,14 @@ struct Scene {
     std::vector<Tri> tris;
     std::vector<int> order;   // leaf order -> triangle id
     std::vector<Node> nodes;
+    // 8-wide version of the same tree for the CPU timing baseline (built on
+    // first use, see build_bvh8)
+    struct WideTree;
+    WideTree* wide = nullptr;
+    Scene() = default;
+    Scene(const Scene&) = delete;
+    Scene& operator=(const Scene&) = delete;
+    ~Scene();
     // vertex indices and the optional per-vertex shading attributes
     std::vector<int32_t> idx;     // 3 per triangle
     std::vector<V3> normals;      // empty: no normals
@@ -1148,14 +1158,174 @@ int orc_render_rows(void* scp, const float* cam, int w, int h, int row0, int row
     return 0;
 }
 
+// ---------------------------------------------------------------- BVH8 (CPU baseline)
+// The CPU timing baseline traverses an 8-wide tree with AVX2, as a stand-in
+// for Embree's 8-wide stream traversal (rtcIntersectNp).  Each wide node's
+// children are BVH2 nodes of the binned-SAH tree: a wide node opens its
+// largest-area internal child until it has 8 children, so every child box is
+// a BVH2 node box and the slab test is the one of closest_fp32_fast, per lane.
+struct WideNode {
+    alignas(32) float lo_x[8], lo_y[8], lo_z[8], hi_x[8], hi_y[8], hi_z[8];
+    int32_t child[8];  // >= 0: wide node; < 0: ~(first << 3 | count), the leaf's triangles in WideTree::tris
+    int32_t n;         // children in slots 0 .. n-1
+};
+
+// The tree's triangles in leaf order (a leaf's <= 4 triangles contiguous),
+// with their original ids.
+struct WideTri {
+    Tri tri;
+    int32_t id;
+};
+
+struct Scene::WideTree {
+    std::vector<WideNode> nodes;
+    std::vector<WideTri> tris;
+};
+
+void build_bvh8(Scene& sc)
+{
+    auto* wt = new Scene::WideTree();
+    sc.wide = wt;
+    if (sc.nodes.empty() || sc.tris.empty()) return;
+    wt->tris.reserve(sc.tris.size());
+    auto area = [&](int i) {
+        const Node& nd = sc.nodes[i];
+        double dx = nd.hi[0] - nd.lo[0], dy = nd.hi[1] - nd.lo[1], dz = nd.hi[2] - nd.lo[2];
+        return dx * dy + dy * dz + dz * dx;
+    };
+    // (wide node index, BVH2 node it stands for)
+    std::vector<std::pair<int, int>> todo{{0, 0}};
+    wt->nodes.push_back(WideNode{});
+    while (!todo.empty()) {
+        auto [wi, b] = todo.back();
+        todo.pop_back();
+        int kids[8];
+        int n = 0;
+        if (sc.nodes[b].left < 0) {
+            kids[n++] = b;  // a leaf root
+        } else {
+            kids[n++] = sc.nodes[b].left;
+            kids[n++] = sc.nodes[b].left + 1;
+            while (n < 8) {
+                int best = -1;
+                double ba = -1.0;
+                for (int k = 0; k < n; ++k)
+                    if (sc.nodes[kids[k]].left >= 0 && area(kids[k]) > ba) { ba = area(kids[k]); best = k; }
+                if (best < 0) break;
+                const int c = kids[best];
+                kids[best] = sc.nodes[c].left;
+                kids[n++] = sc.nodes[c].left + 1;
+            }
+        }
+        WideNode w{};
+        w.n = n;
+        for (int k = 0; k < n; ++k) {
+            const Node& nd = sc.nodes[kids[k]];
+            w.lo_x[k] = nd.lo[0]; w.lo_y[k] = nd.lo[1]; w.lo_z[k] = nd.lo[2];
+            w.hi_x[k] = nd.hi[0]; w.hi_y[k] = nd.hi[1]; w.hi_z[k] = nd.hi[2];
+            if (nd.left < 0) {
+                const int first = (int)wt->tris.size();
+                for (int i = nd.first; i < nd.first + nd.count; ++i)
+                    wt->tris.push_back(WideTri{sc.tris[sc.order[i]], sc.order[i]});
+                w.child[k] = ~((first << 3) | nd.count);
+            } else {
+                w.child[k] = (int)wt->nodes.size();
+                wt->nodes.push_back(WideNode{});
+                todo.push_back({w.child[k], kids[k]});
+            }
+        }
+        wt->nodes[wi] = w;
+    }
+}
+
+// closest_fp32_fast over the 8-wide tree: per visited wide node, the 8 child
+// slab tests in one AVX2 pass (the same sub/mul/min/max per lane as the
+// scalar test), hit children ordered nearest first; leaves test their
+// triangles with the scalar watertight test and the (t, primId) rule.
+__attribute__((target("avx2"))) Closest32 closest_fp32_wide(const Scene& sc, V3 o, V3 d)
+{
+    Closest32 r;
+    r.t = 3.402823466e38f;
+    const std::vector<WideNode>& wide = sc.wide->nodes;
+    const WideTri* wtris = sc.wide->tris.data();
+    if (wide.empty() || sc.tris.empty()) return r;
+    WRay w = wsetup(d);
+    auto sinv = [](float x) { return 1.0f / (std::fabs(x) < 1e-20f ? (x < 0 ? -1e-20f : 1e-20f) : x); };
+    const __m256 ix = _mm256_set1_ps(sinv(d.x)), iy = _mm256_set1_ps(sinv(d.y)), iz = _mm256_set1_ps(sinv(d.z));
+    const __m256 ox = _mm256_set1_ps(o.x), oy = _mm256_set1_ps(o.y), oz = _mm256_set1_ps(o.z);
+    const __m256 zero = _mm256_setzero_ps(), widen = _mm256_set1_ps(1.00000095367431641f);
+    const __m256i lane = _mm256_setr_epi32(0, 1, 2, 3, 4, 5, 6, 7);
+    struct Ent { int ref; float t; };
+    Ent stack[512];
+    int sp = 0;
+    stack[sp++] = {0, 0.0f};
+    while (sp) {
+        const Ent e = stack[--sp];
+        if (e.t > r.t) continue;
+        if (e.ref < 0) {
+            const int first = (~e.ref) >> 3, count = (~e.ref) & 7;
+            for (int i = first; i < first + count; ++i) {
+                const int id = wtris[i].id;
+                float t, vn, wn, det;
+                if (!tri_test_f(w, o, d, wtris[i].tri, 0.0f, r.t, t, vn, wn, det)) continue;
+                if (t < r.t || r.prim < 0 || (t == r.t && id < r.prim)) {
+                    r.t = t; r.prim = id; r.u = vn / det; r.v = wn / det;
+                }
+            }
+            continue;
+        }
+        const WideNode& nd = wide[e.ref];
+        const __m256 ax = _mm256_mul_ps(_mm256_sub_ps(_mm256_load_ps(nd.lo_x), ox), ix);
+        const __m256 bx = _mm256_mul_ps(_mm256_sub_ps(_mm256_load_ps(nd.hi_x), ox), ix);
+        const __m256 ay = _mm256_mul_ps(_mm256_sub_ps(_mm256_load_ps(nd.lo_y), oy), iy);
+        const __m256 by = _mm256_mul_ps(_mm256_sub_ps(_mm256_load_ps(nd.hi_y), oy), iy);
+        const __m256 az = _mm256_mul_ps(_mm256_sub_ps(_mm256_load_ps(nd.lo_z), oz), iz);
+        const __m256 bz = _mm256_mul_ps(_mm256_sub_ps(_mm256_load_ps(nd.hi_z), oz), iz);
+        const __m256 t0 = _mm256_max_ps(_mm256_max_ps(_mm256_min_ps(ax, bx), _mm256_min_ps(ay, by)),
+                                        _mm256_max_ps(_mm256_min_ps(az, bz), zero));
+        const __m256 t1 = _mm256_min_ps(_mm256_min_ps(_mm256_max_ps(ax, bx), _mm256_max_ps(ay, by)),
+                                        _mm256_max_ps(az, bz));
+        const __m256 hit = _mm256_and_ps(_mm256_cmp_ps(t0, _mm256_mul_ps(t1, widen), _CMP_LE_OQ),
+                                         _mm256_cmp_ps(t0, _mm256_set1_ps(r.t), _CMP_LE_OQ));
+        const __m256i valid = _mm256_cmpgt_epi32(_mm256_set1_epi32(nd.n), lane);
+        unsigned mask = (unsigned)_mm256_movemask_ps(_mm256_and_ps(hit, _mm256_castsi256_ps(valid)));
+        if (!mask) continue;
+        alignas(32) float tn[8];
+        _mm256_store_ps(tn, t0);
+        // hit children, farthest first onto the stack (insertion by distance)
+        Ent hits[8];
+        int nh = 0;
+        while (mask) {
+            const int k = __builtin_ctz(mask);
+            mask &= mask - 1;
+            Ent c{nd.child[k], tn[k]};
+            int j = nh++;
+            while (j > 0 && hits[j - 1].t < c.t) { hits[j] = hits[j - 1]; --j; }
+            hits[j] = c;
+        }
+        for (int k = 0; k < nh; ++k) stack[sp++] = hits[k];
+    }
+    return r;
+}
+
+Scene::~Scene() { delete wide; }
+
+bool cpu_has_avx2()
+{
+    static const bool yes = __builtin_cpu_supports("avx2");
+    return yes;
+}
+
 // CPU timing baseline: the stream renderer (DiffuseRendererStream) over the
-// oracle's binned-SAH BVH2, fp32 kernel numerics.  Each thread takes 64x32
+// oracle's binned-SAH tree, fp32 kernel numerics.  Each thread takes 64x32
 // tiles from an atomic counter, generates the tile's rays as one stream,
 // intersects the whole stream, partitions misses before hits, accumulates the
 // misses and shades the hits into the next stream.  Renders `frames` passes
 // of rows [row0, row1); returns rays traced and wall seconds of the frames.
+// simd != 0 (and an AVX2 host): the 8-wide tree with AVX2 child tests
+// (closest_fp32_wide); else the scalar BVH2 walk (closest_fp32_fast).
 int orc_stream_bench(void* scp, const float* cam, int w, int h, int row0, int row1, int pass0, int frames,
-                     int max_depth, float albedo, int threads, float* accum, int64_t* rays_out,
+                     int max_depth, float albedo, int threads, int simd, float* accum, int64_t* rays_out,
                      double* seconds_out)
 {
     const Scene& sc = *static_cast<Scene*>(scp);
@@ -1167,6 +1337,9 @@ int orc_stream_bench(void* scp, const float* cam, int w, int h, int row0, int ro
     row1 = std::min(h, row1);
     int tiles_x = (w + TX - 1) / TX, tiles_y = (row1 - row0 + TY - 1) / TY;
     std::atomic<int64_t> total{0};
+    // the 8-wide tree of the AVX2 traversal, built once per scene, outside the timed region
+    const bool wide = cpu_has_avx2() && simd != 0;
+    if (wide && !sc.wide) build_bvh8(const_cast<Scene&>(sc));
     auto t0 = std::chrono::steady_clock::now();
     for (int f = 0; f < frames; ++f) {
         int pass = pass0 + f;
@@ -1189,7 +1362,7 @@ int orc_stream_bench(void* scp, const float* cam, int w, int h, int row0, int ro
             float lw = 1.0f;
             for (int depth = 0;; ++depth) {
                 for (RayS& r : in) {
-                    Closest32 hc = closest_fp32_fast(sc, r.o, r.d);
+                    Closest32 hc = wide ? closest_fp32_wide(sc, r.o, r.d) : closest_fp32_fast(sc, r.o, r.d);
                     r.prim = hc.prim;
                     r.t = hc.t;
                 }
@@ -1357,5 +1530,6 @@ void orc_paths_adjoint(void* tp, const double* albedo, const double* dldi, doubl
 }
 
 int orc_thread_count(void) { return default_threads(0); }
+int orc_cpu_has_avx2(void) { return cpu_has_avx2() ? 1 : 0; }
 
 }  // extern "C"

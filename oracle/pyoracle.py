@@ -48,10 +48,11 @@ def lib() -> ctypes.CDLL:
         L.orc_render.restype = i
         L.orc_render_rows.argtypes = [P, P, i, i, i, i, i, i, i, f, i, i, i, P, P, P]
         L.orc_render_rows.restype = i
-        L.orc_stream_bench.argtypes = [P, P, i, i, i, i, i, i, i, f, i, P, P, P]
+        L.orc_stream_bench.argtypes = [P, P, i, i, i, i, i, i, i, f, i, i, P, P, P]
         L.orc_stream_bench.restype = i
         L.orc_bounce.argtypes = [P, i64, i, P]
         L.orc_thread_count.restype = i
+        L.orc_cpu_has_avx2.restype = i
         L.orc_paths_record.argtypes = [P, P, i, i, i, i, i, i]
         L.orc_paths_record.restype = P
         L.orc_paths_destroy.argtypes = [P]
@@ -204,13 +205,14 @@ class Scene:
         return {"accum": accum, "sumsq": sumsq, "rays": rays.value}
 
     def stream_bench(self, cam: np.ndarray, w: int, h: int, row0: int, row1: int, frames: int,
-                     max_depth: int, albedo: float, threads: int = 0, pass0: int = 0) -> dict:
+                     max_depth: int, albedo: float, threads: int = 0, pass0: int = 0, simd: bool = True) -> dict:
+        """simd: the 8-wide AVX2 traversal when the host has AVX2, else the scalar BVH2 walk."""
         cam = camera20(cam)
         accum = np.zeros((h, w, 4), dtype=np.float32)
         rays = ctypes.c_int64(0)
         secs = ctypes.c_double(0.0)
         lib().orc_stream_bench(self.h, _p(cam), w, h, row0, row1, pass0, frames, max_depth, albedo, threads,
-                               _p(accum), ctypes.byref(rays), ctypes.byref(secs))
+                               1 if simd else 0, _p(accum), ctypes.byref(rays), ctypes.byref(secs))
         return {"accum": accum, "rays": rays.value, "seconds": secs.value}
 
 
@@ -259,6 +261,10 @@ def bounce(inp: np.ndarray, numerics: int = 0) -> np.ndarray:
     out = np.zeros((inp.shape[0], 6), dtype=np.float32)
     lib().orc_bounce(_p(inp), inp.shape[0], numerics, _p(out))
     return out
+
+
+def cpu_has_avx2() -> bool:
+    return bool(lib().orc_cpu_has_avx2())
 
 
 def thread_count() -> int:

@@ -342,7 +342,8 @@ def test_nearest_of_close_parallel_triangles(orc):
 
 def test_stream_baseline_renders_the_oracle_image(orc, scenes):
     """The CPU timing baseline (restated DiffuseRendererStream over a float
-    BVH2, render/diffuse_renderer_stream.h:82-225) renders the same image as
+    BVH2 or its 8-wide AVX2 form, render/diffuse_renderer_stream.h:82-225)
+    renders the same image as
     the single-ray diffuse oracle (render/diffuse_integrator_single.h:41-69)
     for the same passes.  Its float slab test may cull a box that a ray only
     grazes within rounding (the oracle's walk is conservative), so a few
@@ -352,12 +353,13 @@ def test_stream_baseline_renders_the_oracle_image(orc, scenes):
     v = scenes.CONFIGS["heightfield_box_512"]["view"]
     w, h, frames = 96, 64, 3
     cam = orc.camera_from_view(v["pos"], v["angle_x"], v["angle_y"], v["fov_y"], w / h)
-    st = sc.stream_bench(cam, w, h, 0, h, frames, max_depth=4, albedo=0.8, threads=4)
     ref = sc.render(cam, w, h, spp=frames, max_depth=4, albedo=0.8, renderer=1, numerics=0, threads=4)
-    diff = (st["accum"] != ref["accum"]).any(axis=-1)
-    assert diff.mean() <= 1e-3, f"{int(diff.sum())} of {w * h} pixels differ"
-    assert (st["accum"][:, :, 3] == frames).all()
-    assert abs(st["rays"] - ref["rays"]) <= 4 * int(diff.sum()) + 0
+    for simd in (False, True):  # the scalar BVH2 walk and the 8-wide AVX2 tree (scalar where AVX2 is absent)
+        st = sc.stream_bench(cam, w, h, 0, h, frames, max_depth=4, albedo=0.8, threads=4, simd=simd)
+        diff = (st["accum"] != ref["accum"]).any(axis=-1)
+        assert diff.mean() <= 1e-3, f"simd={simd}: {int(diff.sum())} of {w * h} pixels differ"
+        assert (st["accum"][:, :, 3] == frames).all()
+        assert abs(st["rays"] - ref["rays"]) <= 4 * int(diff.sum())
 
 
 def test_render_rows_equal_full_frame_rows(orc, scenes):
