@@ -303,9 +303,10 @@ int prt_hip_destroy(prt_hip_ctx* ctx)
     void* ptrs[] = {ctx->d_spill,
                     ctx->d_accum, ctx->d_bgr8, ctx->d_rays[0], ctx->d_rays[1], ctx->d_hits,
                     ctx->d_pix[0], ctx->d_pix[1], ctx->d_rng[0], ctx->d_rng[1], ctx->d_counters,
-                    ctx->d_io_rays, ctx->d_io_hits, ctx->d_look, ctx->d_look_side, ctx->d_look_q[0],
-                    ctx->d_look_q[1], ctx->d_look_flags};
+                    ctx->d_io_rays, ctx->d_io_hits, ctx->d_look, ctx->d_look_side, ctx->d_look_flags};
     for (void* p : ptrs) free_ptr(p);
+    for (auto& level : ctx->d_look_q)
+        for (uint4* q : level) free_ptr(q);
     for (hipEvent_t e : ctx->kev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -622,11 +623,13 @@ int prt_hip_set_lookahead(prt_hip_ctx* ctx, int enable)
 
 namespace {
 
-// Buffers of the frame lookahead: the state, one side entry per pixel and two
-// queues (read by this call, written for the next) of one path per lane.
+// Buffers of the frame lookahead: the state, kLookLevels side entries per
+// pixel and, per level, two queues (read by this call, written for the next)
+// of one path per lane.
 int ensure_look_resources(prt_hip_ctx* ctx)
 {
     const int64_t pixels = (int64_t)ctx->width * ctx->height;
+    const int64_t side_len = pixels * kLookLevels;
     const uint32_t cap = (uint32_t)ctx->trace_grid * (uint32_t)kTraceBlock;
     bool fresh = false;
     if (!ctx->d_look) {
@@ -634,24 +637,26 @@ int ensure_look_resources(prt_hip_ctx* ctx)
         PRT_TRY(ctx, hipMemsetAsync(ctx->d_look, 0, sizeof(LookState), ctx->stream));
         fresh = true;
     }
-    if (pixels != ctx->look_side_len) {
+    if (side_len != ctx->look_side_len) {
         free_ptr(ctx->d_look_side);
         ctx->d_look_side = nullptr;
         ctx->look_side_len = 0;
-        PRT_TRY(ctx, hipMalloc(&ctx->d_look_side, sizeof(uint2) * pixels));
-        PRT_TRY(ctx, hipMemsetAsync(ctx->d_look_side, 0, sizeof(uint2) * pixels, ctx->stream));
-        ctx->look_side_len = pixels;
+        PRT_TRY(ctx, hipMalloc(&ctx->d_look_side, sizeof(uint2) * side_len));
+        PRT_TRY(ctx, hipMemsetAsync(ctx->d_look_side, 0, sizeof(uint2) * side_len, ctx->stream));
+        ctx->look_side_len = side_len;
         fresh = true;
     }
     if (cap > ctx->look_q_cap) {
-        for (uint4*& q : ctx->d_look_q) {
-            free_ptr(q);
-            q = nullptr;
-        }
+        for (auto& level : ctx->d_look_q)
+            for (uint4*& q : level) {
+                free_ptr(q);
+                q = nullptr;
+            }
         free_ptr(ctx->d_look_flags);
         ctx->d_look_flags = nullptr;
         ctx->look_q_cap = 0;
-        for (uint4*& q : ctx->d_look_q) PRT_TRY(ctx, hipMalloc(&q, 3 * sizeof(uint4) * (size_t)cap));
+        for (auto& level : ctx->d_look_q)
+            for (uint4*& q : level) PRT_TRY(ctx, hipMalloc(&q, 3 * sizeof(uint4) * (size_t)cap));
         const size_t flag_bytes = sizeof(uint32_t) * kLookFlagStride * (size_t)ctx->trace_grid;
         PRT_TRY(ctx, hipMalloc(&ctx->d_look_flags, flag_bytes));
         PRT_TRY(ctx, hipMemsetAsync(ctx->d_look_flags, 0, flag_bytes, ctx->stream));
@@ -754,12 +759,17 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
         p.fast = fast;
         p.coop = ctx->tail_coop;
         if (look_run) {
-            ctx->look_gen += 1;
+            // a call that keeps no earlier work skips kLookLevels tags, so no
+            // side sample left by an earlier chain of calls carries its tag
+            // or a later one's
+            ctx->look_gen += look_valid ? 1u : (uint32_t)kLookLevels + 1u;
             PRT_TRY(ctx, launch_look_call_begin(ctx, look_valid, entries));
             p.look = ctx->d_look;
             p.side = ctx->d_look_side;
-            p.q_in = ctx->d_look_q[(ctx->look_gen - 1) & 1];
-            p.q_out = ctx->d_look_q[ctx->look_gen & 1];
+            for (int l = 0; l < kLookLevels; ++l) {
+                p.q_in[l] = ctx->d_look_q[l][(ctx->look_gen - 1) & 1];
+                p.q_out[l] = ctx->d_look_q[l][ctx->look_gen & 1];
+            }
             p.q_cap = ctx->look_q_cap;
             p.look_gen = ctx->look_gen;
             p.look_flags = ctx->d_look_flags;

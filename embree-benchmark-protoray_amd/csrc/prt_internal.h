@@ -157,6 +157,7 @@ struct BatchTuning {
 };
 
 struct LookState;
+constexpr int kLookLevels = 2;  // frames past its own that a diffuse call may start
 
 }  // namespace prt
 
@@ -237,9 +238,9 @@ struct prt_hip_ctx {
     int lookahead = 1;
     uint64_t scene_gen = 0;             // bumped by every change of the plain scene
     prt::LookState* d_look = nullptr;
-    uint2* d_look_side = nullptr;       // one entry per pixel
+    uint2* d_look_side = nullptr;       // kLookLevels slots of one entry per pixel
     int64_t look_side_len = 0;
-    uint4* d_look_q[2] = {nullptr, nullptr};
+    uint4* d_look_q[prt::kLookLevels][2] = {};  // per level, read by this call / written for the next
     uint32_t look_q_cap = 0;
     uint32_t* d_look_flags = nullptr;   // one stop flag per block of the launch
     uint32_t look_gen = 0;
@@ -278,22 +279,29 @@ struct TraceLaunch {
 };
 // Fused path launch (one frame, one sample per pixel)
 // Frame lookahead of the fused diffuse renderer: device state that one render
-// call hands to the next.  A call renders frame p: first the paths of frame p
-// that the previous call suspended, then the items of frame p it did not
-// start; once those are all handed out, idle lanes start items of frame p + 1.
-// When the last path of frame p finishes, next-frame paths stop at their next
-// bounce and are queued for the next call; those that finished already wait
-// in a per-pixel side buffer, tagged with the call that wrote them.
+// call hands to the next.  A call renders frame p; a path of frame p + l is
+// "at level l".  Work items, in the order they are handed out:
+//  - this frame's space: the items of frame p that earlier calls did not
+//    start (S_0 of them were), then the paths that the previous call
+//    suspended, level by level: R_0 of frame p, R_1 of frame p + 1, ...;
+//  - once those are all handed out, the later frames' space: the items of
+//    frame p + 1 not started yet, then those of p + 2, ... up to
+//    p + kLookLevels (each level's items in path order, from S_l on).
+// When the last path of frame p finishes, paths of later frames stop at their
+// next bounce (or in the middle of a ray, which is traced again) and are
+// queued for the next call, where level l becomes level l - 1; those that
+// finished wait in a per-pixel side buffer, tagged with the call that adds
+// them to the accumulator.
 struct LookState {
-    uint32_t counter;       // work counter over this frame's virtual items [0, R + entries - S)
-    uint32_t spec_counter;  // work counter over the next frame's items
-    uint32_t resume_in;     // R: suspended paths of this frame in the input queue
-    uint32_t started;       // S: items of this frame the previous call started
-    uint32_t target;        // paths of this frame left: R + (entries - S)
+    uint32_t counter;       // work counter over this frame's space
+    uint32_t spec_counter;  // work counter over the later frames' space
+    uint32_t target;        // items of this frame's space (0: the frame is finished)
     uint32_t live;          // waves that hold, or can still take, paths of this frame
-    uint32_t resume_out;    // next-frame paths suspended by this call
-    uint32_t gen;           // tag of this call's side samples and stop flags
-    unsigned long long spec_rays;  // rays of the next frame traced by this call
+    uint32_t started[kLookLevels + 1];     // S_l: items of frame p + l that earlier calls started (S_K = 0)
+    uint32_t resume_in[kLookLevels + 1];   // R_l: paths of frame p + l in input queue l (R_K = 0)
+    uint32_t spec_begin[kLookLevels + 2];  // level l >= 1 owns [spec_begin[l], spec_begin[l + 1]) of spec_counter
+    uint32_t resume_out[kLookLevels + 1];  // paths of frame p + l (l >= 1) that this call suspended
+    unsigned long long rays[kLookLevels + 1];  // rays of frame p + l (l >= 1) traced by this call and earlier ones
 };
 constexpr int kLookFlagStride = 16;  // u32 per block stop flag (one 64 B line each)
 
@@ -328,10 +336,10 @@ struct PathLaunch {
     int coop;
     // diffuse frames: frame lookahead (nullptr: off)
     LookState* look;
-    uint2* side;          // per pixel: (next-frame sample bits, tag)
-    const uint4* q_in;    // suspended paths of this frame, 3 uint4 each
-    uint4* q_out;         // suspended paths of the next frame
-    uint32_t q_cap;       // capacity of q_out in paths (>= lanes of the launch)
+    uint2* side;          // kLookLevels slots of W*H: (sample bits, tag of the call that adds it)
+    const uint4* q_in[kLookLevels];   // suspended paths of frame p + l, 3 uint4 each
+    uint4* q_out[kLookLevels];        // suspended paths of frame p + l + 1 (the next call's level l)
+    uint32_t q_cap;       // capacity of each queue in paths (>= lanes of the launch)
     uint32_t look_gen;    // this call's tag
     uint32_t* look_flags; // per block (kLookFlagStride apart): look_gen once this frame is finished
 };

@@ -89,14 +89,16 @@ __device__ __forceinline__ void accum_add(float4* accum, int pixel, float c, boo
 
 // Set-up of a lookahead call, in one launch before the path kernel.
 //  - Block 0 clears the counters and statistics (as k_frame_begin), then
-//    hands the state that the previous call left to this call (LookState).
-//    valid: the previous call rendered the previous pass of the same frame
-//    with lookahead on; else its next-frame work is dropped.  The previous
-//    call took next-frame items in order from its spec_counter, so the items
-//    it started are a prefix of this frame's.
+//    hands the state that the previous call left to this call (LookState):
+//    level l of the previous call is level l - 1 here.  valid: the previous
+//    call rendered the previous pass with the same inputs and lookahead on;
+//    else every path it started for later frames is dropped.  The previous
+//    call took each level's items in path order from its share of
+//    spec_counter, so the items of a frame started so far are a prefix.
 //  - With a valid hand-over, every block adds the samples of this frame that
-//    the previous call finished (tagged `tag`) to its pixels' accumulators;
-//    those pixels get no other sample in this call.
+//    earlier calls finished (side slot tag % kLookLevels, tagged `tag`) to
+//    their pixels' accumulators; those pixels get no other sample in this
+//    call.
 __global__ __launch_bounds__(kBlock) void k_look_call_begin(uint32_t* counters, uint32_t n0,
                                                             unsigned long long* stats, LookState* ls,
                                                             uint32_t entries, int valid, uint32_t q_cap,
@@ -108,28 +110,34 @@ __global__ __launch_bounds__(kBlock) void k_look_call_begin(uint32_t* counters, 
         if (threadIdx.x < 32) stats[threadIdx.x] = threadIdx.x == 9 ? ~0ull : 0ull;  // [9]: a minimum
         __syncthreads();
         if (threadIdx.x == 0) {
-            uint32_t r = 0, s = 0;
+            LookState nx{};
             unsigned long long carried = 0;
             if (valid) {
-                s = min(entries, ls->spec_counter);
-                r = min(ls->resume_out, q_cap);
-                carried = ls->spec_rays;
+                const uint32_t claimed = ls->spec_counter;
+                for (int l = 1; l <= kLookLevels; ++l) {
+                    const uint32_t begin = ls->spec_begin[l], end = ls->spec_begin[l + 1];
+                    const uint32_t got = claimed > begin ? min(claimed, end) - begin : 0u;
+                    nx.started[l - 1] = ls->started[l] + got;
+                    nx.resume_in[l - 1] = min(ls->resume_out[l], q_cap);
+                    if (l < kLookLevels) nx.rays[l] = ls->rays[l + 1];
+                }
+                carried = ls->rays[1];
             }
-            ls->counter = 0;
-            ls->spec_counter = 0;
-            ls->resume_in = r;
-            ls->started = s;
-            ls->target = r + (entries - s);
-            ls->live = waves;
-            ls->resume_out = 0;
-            ls->spec_rays = 0;
+            // this frame's space: its items not started yet, then the
+            // suspended paths level by level
+            uint32_t items = entries - nx.started[0];
+            for (int l = 0; l < kLookLevels; ++l) items += nx.resume_in[l];
+            nx.target = items;
+            for (int l = 1; l <= kLookLevels; ++l) nx.spec_begin[l + 1] = nx.spec_begin[l] + (entries - nx.started[l]);
+            nx.live = waves;
+            *ls = nx;
             stats[3] = carried;
         }
     }
     if (valid) {
         const int i = blockIdx.x * kBlock + threadIdx.x;
         if (i < n) {
-            const uint2 e = side[i];
+            const uint2 e = side[(size_t)(tag % (uint32_t)kLookLevels) * (size_t)n + i];
             if (e.y == tag) accum_add(accum, i, __uint_as_float(e.x), false);
         }
     }
@@ -381,7 +389,7 @@ hipError_t launch_look_call_begin(prt_hip_ctx* ctx, bool valid, uint32_t entries
     k_look_call_begin<<<grid, kBlock, 0, ctx->stream>>>(
         ctx->d_counters, entries, ctx->d_stats, ctx->d_look, entries, valid ? 1 : 0, ctx->look_q_cap,
         (uint32_t)ctx->diffuse_grid * (uint32_t)(kTraceBlock / 64), ctx->d_look_side, ctx->d_accum, n,
-        ctx->look_gen - 1);
+        ctx->look_gen);
     return hipGetLastError();
 }
 

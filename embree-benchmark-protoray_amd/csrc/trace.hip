@@ -146,35 +146,47 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     const bool coop_on = kCoop && pl.coop != 0;
     bool cont = false;  // the path's next ray waits in (o, d) for the cooperative phase
 
-    // Frame lookahead (kModeDiffuse, LookState).  This frame's virtual items,
-    // from LookState::counter: [0, F) are its items S..entries-1 (F =
-    // entries - S), [F, fresh_end) resume the R paths the previous call
-    // suspended (they are part way through, so they go last).
-    //  - A wave that has seen all of them handed out takes items of the next
-    //    frame from LookState::spec_counter, but only while it holds no path
-    //    of this frame: waves that do take nothing more, stay sparse and
-    //    issue first (s_setprio).
+    // Frame lookahead (kModeDiffuse, LookState); a path of frame pass + l is
+    // at level l.  This frame's space, from LookState::counter: [0, F) are
+    // its items S_0..entries-1 (F = entries - S_0), then the paths the
+    // previous call suspended, level by level: queue l holds lk_res[l + 1] -
+    // lk_res[l] of them from lk_res[l] on (they are part way through, so
+    // they go last).
+    //  - A wave that has seen all of them handed out takes items of the
+    //    later frames from LookState::spec_counter (level l: path S_l + j
+    //    for the j-th item of its share), but only while it holds no path of
+    //    this frame: waves that do take nothing more, stay sparse and issue
+    //    first (s_setprio).
     //  - Every wave starts registered in LookState::live and leaves it once,
-    //    at that same point (all handed out, none held); the last to leave
-    //    writes this call's tag into each block's stop flag.
-    //  - After the stop no more items are taken, and next-frame paths leave
-    //    for the queue: at a bounce, or in the middle of a ray, which the
-    //    next call then traces again from its origin.
+    //    at that same point (all handed out, no level-0 path held); the last
+    //    to leave writes this call's tag into each block's stop flag.
+    //  - After the stop no more items are taken, and paths of later frames
+    //    leave for the queue of their level: at a bounce, or in the middle
+    //    of a ray, which the next call then traces again from its origin.
     //  - The cooperative tail is off under the lookahead.
     constexpr bool kLookMode = kMode == kModeDiffuse && kLook;
     LookState* const look = kLookMode ? pl.look : nullptr;
-    uint32_t lk_F = 0, lk_S = 0, lk_fresh_end = 0, lk_target = 0;
+    uint32_t lk_F = 0, lk_fresh_end = 0, lk_target = 0;
+    uint32_t lk_S[kLookLevels + 1] = {};         // S_l
+    uint32_t lk_res[kLookLevels + 1] = {};       // this frame's space: queue l from lk_res[l]
+    uint32_t lk_spec[kLookLevels + 2] = {};      // later frames' space: level l from lk_spec[l]
     if (kLookMode && look) {
-        lk_F = count - look->started;
-        lk_S = look->started;
+#pragma unroll
+        for (int l = 0; l <= kLookLevels; ++l) lk_S[l] = look->started[l];
+        lk_F = count - lk_S[0];
         lk_target = look->target;
-        lk_fresh_end = lk_F + look->resume_in;
+        lk_res[0] = lk_F;
+#pragma unroll
+        for (int l = 0; l < kLookLevels; ++l) lk_res[l + 1] = lk_res[l] + look->resume_in[l];
+        lk_fresh_end = lk_res[kLookLevels];
+#pragma unroll
+        for (int l = 1; l <= kLookLevels + 1; ++l) lk_spec[l] = look->spec_begin[l];
     }
     bool stop = kLookMode && look && lk_target == 0;  // wave-uniform: this frame is finished
-    bool fresh_out = false;  // wave-uniform: every item of this frame is handed out
+    bool fresh_out = false;  // wave-uniform: every item of this frame's space is handed out
     bool live = kLookMode && look;  // wave-uniform: registered in LookState::live
-    bool spec = false;       // the lane's path belongs to the next frame
-    bool suspend = false;    // the lane's next-frame path waits in (o, d) for the queue
+    int level = 0;           // the lane's path belongs to frame pass + level
+    bool suspend = false;    // the lane's later-frame path waits in (o, d) for the queue
     uint32_t poll_steps = 0;
     uint32_t* const stop_flag = kLookMode && look ? pl.look_flags + (size_t)blockIdx.x * kLookFlagStride : nullptr;
     // leaves LookState::live; the last wave to leave raises every stop flag
@@ -188,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                                    __HIP_MEMORY_SCOPE_AGENT);
         live = false;
     };
-    unsigned long long n_spec_rays = 0;
+    unsigned long long n_ahead[kLookLevels] = {};  // rays of frame pass + l + 1
     if (stop) exhausted = true;
 
     auto start_ray = [&](f3 org, f3 dir, float near, float far) {
@@ -239,20 +251,20 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     };
 
     for (;;) {
-        // lookahead: a wave that holds paths of this frame takes no next-frame
-        // items once this frame's are handed out, and issues ahead of the
-        // waves that run next-frame paths, so the frame's last paths keep the
-        // pace they would have alone
+        // lookahead: a wave that holds paths of this frame takes no items of
+        // later frames once this frame's are handed out, and issues ahead of
+        // the waves that run later frames' paths, so the frame's last paths
+        // keep the pace they would have alone
         bool hold = false;
         if (kLookMode && look) {
-            hold = fresh_out && __ballot(active && !spec) != 0;
+            hold = fresh_out && __ballot(active && level == 0) != 0;
             if (live && fresh_out && !hold) look_leave();
             if (!live && !stop) {
                 const uint32_t f = __hip_atomic_load(stop_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 stop = (uint32_t)__builtin_amdgcn_readfirstlane(f) == pl.look_gen;
                 if (stop) exhausted = true;
             }
-            // after the stop only next-frame paths are left: those in the
+            // after the stop only later frames' paths are left: those in the
             // middle of a ray leave now and trace that ray again next call
             // (its origin, direction and path state are what a bounce leaves)
             if (stop && tracing) {
@@ -306,8 +318,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                 break;
             }
             if constexpr (kCount && kMode != kModeBatch) atomicAdd(&depth_hist[min(depth, kMaxQueueDepth - 1)], 1u);
-            if (kLookMode && spec) ++n_spec_rays;
-            else ++n_rays;
+            if (kLookMode && level > 0) {
+#pragma unroll
+                for (int l = 1; l <= kLookLevels; ++l)
+                    if (level == l) ++n_ahead[l - 1];
+            } else {
+                ++n_rays;
+            }
             float4* acc = kMode == kModeBatch ? nullptr : pl.accum + pixel;
             if (kMode == kModeDiffuse2) {
                 // extension ray done (render/diffuse2_integrator_single.h:52-85)
@@ -383,8 +400,11 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                         const float4 a = *acc;
                         *acc = make_float4(a.x + c, a.y + cg, a.z + cb, a.w + 1.0f);
                     }
-                } else if (kLookMode && spec) {
-                    pl.side[pixel] = make_uint2(__float_as_uint(c), pl.look_gen);
+                } else if (kLookMode && level > 0) {
+                    // a later frame's sample, for the call whose tag it carries
+                    const uint32_t tag = pl.look_gen + (uint32_t)level;
+                    pl.side[(size_t)(tag % (uint32_t)kLookLevels) * (size_t)(pl.width * pl.height) + pixel] =
+                        make_uint2(__float_as_uint(c), tag);
                 } else {
                     const float4 a = pl.fast ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : *acc;
                     *acc = make_float4(a.x + c, a.y + c, a.z + c, a.w + 1.0f);
@@ -427,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             f3 no, nd;
             diffuse_bounce(o, d, t, fr, s1, s2, no, nd);
             ++depth;
-            if (kLookMode && spec && stop) {
+            if (kLookMode && level > 0 && stop) {
                 o = no;
                 d = nd;
                 suspend = true;
@@ -444,18 +464,21 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
         } while (0);
         if (kCoop && go_coop) break;
 
-        // ---- lookahead: queue the next-frame paths that stopped at a bounce
-        if (kLookMode && look) {
-            const uint64_t sm = __ballot(suspend);
-            if (sm != 0) {
+        // ---- lookahead: queue the later frames' paths that stopped, by level
+        if (kLookMode && look && __ballot(suspend) != 0) {
+#pragma unroll
+            for (int l = 1; l <= kLookLevels; ++l) {
+                const bool mine = suspend && level == l;
+                const uint64_t sm = __ballot(mine);
+                if (sm == 0) continue;
                 const int leader = __ffsll((unsigned long long)sm) - 1;
                 uint32_t qb = 0;
-                if (lane == leader) qb = atomicAdd(&look->resume_out, (uint32_t)__popcll(sm));
+                if (lane == leader) qb = atomicAdd(&look->resume_out[l], (uint32_t)__popcll(sm));
                 qb = __shfl(qb, leader, 64);
-                if (suspend) {
+                if (mine) {
                     const uint32_t slot = qb + (uint32_t)__popcll(sm & lane_lt);
                     if (slot < pl.q_cap) {
-                        uint4* q = pl.q_out + 3 * (size_t)slot;
+                        uint4* q = pl.q_out[l - 1] + 3 * (size_t)slot;
                         q[0] = make_uint4(__float_as_uint(o.x), __float_as_uint(o.y), __float_as_uint(o.z),
                                           (uint32_t)pixel);
                         q[1] = make_uint4(__float_as_uint(d.x), __float_as_uint(d.y), __float_as_uint(d.z), rng);
@@ -463,9 +486,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                     } else {
                         atomicOr(err, 2u);
                     }
-                    suspend = false;
                 }
             }
+            suspend = false;
         }
 
         // ---- refill idle lanes (wave-uniform branch)
@@ -475,8 +498,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
             const int leader = __ffsll((unsigned long long)idle) - 1;
             uint32_t* ctr = kRayQuery ? tl.fetch_ptr : (kMode == kModeBatch ? bl.work_counter : pl.path_counter);
             uint32_t lim = count;
-            const bool spec_claim = kLookMode && look && fresh_out;  // next-frame items
-            if (kLookMode && look && !spec_claim) lim = lk_fresh_end;
+            const bool spec_claim = kLookMode && look && fresh_out;  // items of later frames
+            if (kLookMode && look) lim = spec_claim ? lk_spec[kLookLevels + 1] : lk_fresh_end;
             if (spec_claim) ctr = &look->spec_counter;
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(ctr, (uint32_t)n_idle);
@@ -537,15 +560,21 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                             active = true;
                         }
                     } else if (kLookMode && look && !spec_claim && my >= lk_F) {
-                        // a path of this frame that the previous call stopped
-                        // at a bounce: its next ray
-                        const uint4* q = pl.q_in + 3 * (size_t)(my - lk_F);
+                        // a path that the previous call stopped at a bounce
+                        // (or in the middle of a ray): its next ray
+                        const uint4* q = pl.q_in[0] + 3 * (size_t)(my - lk_res[0]);
+                        level = 0;
+#pragma unroll
+                        for (int l = 1; l < kLookLevels; ++l)
+                            if (my >= lk_res[l]) {
+                                q = pl.q_in[l] + 3 * (size_t)(my - lk_res[l]);
+                                level = l;
+                            }
                         const uint4 e0 = q[0], e1 = q[1], e2 = q[2];
                         pixel = (int32_t)e0.w;
                         rng = e1.w;
                         depth = (int)e2.x;
                         lw = __uint_as_float(e2.y);
-                        spec = false;
                         start_ray(mk3(as_f(e0.x), as_f(e0.y), as_f(e0.z)), mk3(as_f(e1.x), as_f(e1.y), as_f(e1.z)),
                                   0.0f, 3.402823466e38f);
                         active = true;
@@ -555,9 +584,17 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                         uint32_t path = my;
                         path_pass = pl.pass;
                         if (kLookMode && look) {
-                            spec = spec_claim;
-                            path = spec ? my : lk_S + my;
-                            path_pass = pl.pass + (spec ? 1 : 0);
+                            level = 0;
+                            path = lk_S[0] + my;
+                            if (spec_claim) {
+#pragma unroll
+                                for (int l = 1; l <= kLookLevels; ++l)
+                                    if (my >= lk_spec[l]) {
+                                        level = l;
+                                        path = lk_S[l] + (my - lk_spec[l]);
+                                    }
+                            }
+                            path_pass = pl.pass + level;
                         }
                         if (kMode == kModeAdjoint || (kMode == kModeRgb && pl.rgb_planes)) {
                             const uint32_t sample = my / pl.entries;
@@ -1165,9 +1202,12 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
     }
     if constexpr (kLookMode) {
         if (look) {
-            unsigned long long r = n_spec_rays;
-            for (int off = 32; off > 0; off >>= 1) r += __shfl_down(r, off, 64);
-            if (lane == 0 && r) atomicAdd(&look->spec_rays, r);
+#pragma unroll
+            for (int l = 0; l < kLookLevels; ++l) {
+                unsigned long long r = n_ahead[l];
+                for (int off = 32; off > 0; off >>= 1) r += __shfl_down(r, off, 64);
+                if (lane == 0 && r) atomicAdd(&look->rays[l + 1], r);
+            }
         }
     }
     if constexpr (!kRayQuery) {
@@ -1237,9 +1277,15 @@ hipError_t launch_batch_variant(prt_hip_ctx* ctx, const BatchLaunch& bl, int blo
 int ensure_trace_resources(prt_hip_ctx* ctx)
 {
     if (ctx->trace_per_cu == 0) {
-        int per_cu = 0;
+        // the lookahead variant's workgroups must all be resident at once: a
+        // workgroup that starts only when another ends would hold the frame's
+        // stop (every wave leaves the live count) until then
+        int per_cu = 0, per_cu_look = 0;
         PRT_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, kModeDiffuse>,
                                                                   kTraceBlock, 0));
+        PRT_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                         &per_cu_look, k_trace<false, kModeDiffuse, kTraceBlock, kStackLds, true>, kTraceBlock, 0));
+        if (per_cu_look < per_cu) per_cu = per_cu_look;
         if (per_cu * kTraceBlock > 2048) per_cu = 2048 / kTraceBlock;
         if (per_cu < 1) per_cu = 1;
         ctx->trace_per_cu = per_cu;
