@@ -77,6 +77,14 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
+    // LDS ring entry q of this lane: byte offset ((q mod kStack) * kBlock + tid) * 4
+    constexpr int kRingShift = kBlock == 256 ? 10 : (kBlock == 128 ? 9 : 8);
+    static_assert(kBlock == 256 || kBlock == 128 || kBlock == 64, "ring layout");
+    constexpr uint32_t kRingMask = (uint32_t)(kStack - 1) << kRingShift;
+    const uint32_t tid4 = (uint32_t)tid * 4u;
+    auto ring_off = [&](int q) -> uint32_t { return (((uint32_t)q << kRingShift) & kRingMask) | tid4; };
+    auto ring_node = [&](uint32_t off) -> uint32_t& { return *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(st_node) + off); };
+    auto ring_t = [&](uint32_t off) -> float& { return *reinterpret_cast<float*>(reinterpret_cast<char*>(st_t) + off); };
     const uint64_t lane_lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint32_t count = kRayQuery ? (tl.count_ptr ? *tl.count_ptr : tl.count)
                          : (kMode == kModeBatch ? bl.total_work
@@ -725,9 +733,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                             nid = e.x;
                             nt = as_f(e.y);
                         } else {
-                            const int slot = (sp & (kStack - 1)) * kBlock + tid;
-                            nid = st_node[slot];
-                            nt = st_t[slot];
+                            const uint32_t off = ring_off(sp);
+                            nid = ring_node(off);
+                            nt = ring_t(off);
                         }
                         if (nt <= tbest) { node = (int32_t)nid; break; }
                     }
@@ -754,13 +762,18 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                 const float bx = (as_f(w0.x) - o.x) * inv_x;
                 const float by = (as_f(w0.y) - o.y) * inv_y;
                 const float bz = (as_f(w0.z) - o.z) * inv_z;
-                // rounding slack of fma(q, a, b): 2^-21 (255|a| + |b|)
-                const float slx = 4.76837158203125e-7f * (255.0f * fabsf(ax) + fabsf(bx));
-                const float sly = 4.76837158203125e-7f * (255.0f * fabsf(ay) + fabsf(by));
-                const float slz = 4.76837158203125e-7f * (255.0f * fabsf(az) + fabsf(bz));
-                const float bxn = bx - slx, bxf = bx + slx;
-                const float byn = by - sly, byf = by + sly;
-                const float bzn = bz - slz, bzf = bz + slz;
+                // A plane byte q enters as the half 1024 + q (two per v_perm,
+                // read by v_fma_mix_f32 without a conversion), so the distance
+                // is fma(1024 + q, a, b' - 1024 a), b' = b -/+ slack.  Slack
+                // 2^-21 (512|a| + 2|b|) covers the rounding of b, of b' and of
+                // b' - 1024 a and of the fma, whose operands now reach
+                // 1279|a| + |b'| + 1024|a|, with room to spare.
+                const float slx = 4.76837158203125e-7f * (512.0f * fabsf(ax) + 2.0f * fabsf(bx));
+                const float sly = 4.76837158203125e-7f * (512.0f * fabsf(ay) + 2.0f * fabsf(by));
+                const float slz = 4.76837158203125e-7f * (512.0f * fabsf(az) + 2.0f * fabsf(bz));
+                const float bxn = __builtin_fmaf(-1024.0f, ax, bx - slx), bxf = __builtin_fmaf(-1024.0f, ax, bx + slx);
+                const float byn = __builtin_fmaf(-1024.0f, ay, by - sly), byf = __builtin_fmaf(-1024.0f, ay, by + sly);
+                const float bzn = __builtin_fmaf(-1024.0f, az, bz - slz), bzf = __builtin_fmaf(-1024.0f, az, bz + slz);
                 // near/far planes per axis from the direction signs
                 const uint32_t nx0 = neg_x ? w3.z : w2.x, nx1 = neg_x ? w3.w : w2.y;
                 const uint32_t fx0 = neg_x ? w2.x : w3.z, fx1 = neg_x ? w2.y : w3.w;
@@ -769,6 +782,21 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                 const uint32_t nz0 = neg_z ? w4.z : w3.x, nz1 = neg_z ? w4.w : w3.y;
                 const uint32_t fz0 = neg_z ? w3.x : w4.z, fz1 = neg_z ? w3.y : w4.w;
                 const uint32_t meta0 = w1.z, meta1 = w1.w;
+                // the halves 1024 + q of plane word w: bytes 0, 1 (lo) and 2, 3 (hi)
+                auto halves = [](uint32_t w) {
+                    return make_uint2(__builtin_amdgcn_perm(0x64646464u, w, 0x04010400u),
+                                      __builtin_amdgcn_perm(0x64646464u, w, 0x04030402u));
+                };
+                const uint2 hnx0 = halves(nx0), hnx1 = halves(nx1), hfx0 = halves(fx0), hfx1 = halves(fx1);
+                const uint2 hny0 = halves(ny0), hny1 = halves(ny1), hfy0 = halves(fy0), hfy1 = halves(fy1);
+                const uint2 hnz0 = halves(nz0), hnz1 = halves(nz1), hfz0 = halves(fz0), hfz1 = halves(fz1);
+                // 1024 + byte k of the plane word pair (h0: bytes 0-3, h1: bytes 4-7)
+                auto qf = [](uint2 h0, uint2 h1, int k) -> float {
+                    const uint2 h = k < 4 ? h0 : h1;
+                    const uint32_t w = (k & 2) ? h.y : h.x;
+                    const unsigned short bits = (unsigned short)((k & 1) ? (w >> 16) : (w & 0xffffu));
+                    return (float)__builtin_bit_cast(_Float16, bits);
+                };
 
                 // branchless child tests; empty slots carry inverted boxes, and a
                 // leaf slot with meta 0 holds no triangle, so no leaf mask is needed
@@ -776,16 +804,12 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                 float tk[8];
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
-                    const int kb = k & 3;
-                    const uint32_t nxw = k < 4 ? nx0 : nx1, fxw = k < 4 ? fx0 : fx1;
-                    const uint32_t nyw = k < 4 ? ny0 : ny1, fyw = k < 4 ? fy0 : fy1;
-                    const uint32_t nzw = k < 4 ? nz0 : nz1, fzw = k < 4 ? fz0 : fz1;
-                    const float tnx = __builtin_fmaf(byte_f(nxw, kb), ax, bxn);
-                    const float tny = __builtin_fmaf(byte_f(nyw, kb), ay, byn);
-                    const float tnz = __builtin_fmaf(byte_f(nzw, kb), az, bzn);
-                    const float tfx = __builtin_fmaf(byte_f(fxw, kb), ax, bxf);
-                    const float tfy = __builtin_fmaf(byte_f(fyw, kb), ay, byf);
-                    const float tfz = __builtin_fmaf(byte_f(fzw, kb), az, bzf);
+                    const float tnx = __builtin_fmaf(qf(hnx0, hnx1, k), ax, bxn);
+                    const float tny = __builtin_fmaf(qf(hny0, hny1, k), ay, byn);
+                    const float tnz = __builtin_fmaf(qf(hnz0, hnz1, k), az, bzn);
+                    const float tfx = __builtin_fmaf(qf(hfx0, hfx1, k), ax, bxf);
+                    const float tfy = __builtin_fmaf(qf(hfy0, hfy1, k), ay, byf);
+                    const float tfz = __builtin_fmaf(qf(hfz0, hfz1, k), az, bzf);
                     const float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
                     const float tf = fminf(fminf(fminf(tfx, tfy), tfz), tbest);
                     tk[k] = tn;
@@ -827,10 +851,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                     // slot; node_t is the key's distance, at most 7 ulps below
                     // the true one, so culling with it stays conservative.
                     uint32_t key[8];
+                    const uint32_t not_int = ~hit_int;
 #pragma unroll
-                    for (int k = 0; k < 8; ++k)
-                        key[k] = ((hit_int >> k) & 1u) ? ((__float_as_uint(tk[k]) & 0x7ffffff8u) | (uint32_t)k)
-                                                      : 0xffffffffu;
+                    for (int k = 0; k < 8; ++k) {
+                        // all ones unless bit k of hit_int is set (a 1-bit signed field)
+                        const uint32_t miss = (uint32_t)__builtin_amdgcn_sbfe((int)not_int, (unsigned)k, 1u);
+                        key[k] = (__float_as_uint(tk[k]) & 0x7ffffff8u) | (uint32_t)k | miss;
+                    }
                     const uint32_t m1 = min(min(min(key[0], key[1]), min(key[2], key[3])),
                                             min(min(key[4], key[5]), min(key[6], key[7])));
                     // second smallest: no other key equals m1, so key - m1 - 1
@@ -854,8 +881,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
                         bool ok = true;
                         while (sp + npush - bottom > kStack) {
                             if (bottom >= spill_cap) { atomicOr(err, 1u); ok = false; break; }
-                            const int bslot = (bottom & (kStack - 1)) * kBlock + tid;
-                            my_spill[bottom] = make_uint2(st_node[bslot], __float_as_uint(st_t[bslot]));
+                            const uint32_t boff = ring_off(bottom);
+                            my_spill[bottom] = make_uint2(ring_node(boff), __float_as_uint(ring_t(boff)));
                             ++bottom;
                             if (kCount) ++n_spill;
                         }
@@ -871,15 +898,15 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDevice sc, TraceLaunch tl
 #pragma unroll
                             for (int k = 0; k < 8; ++k) {
                                 const int pos = __popc(pm2 & ((1u << k) - 1u));
-                                const int slot = ((sp + pos) & (kStack - 1)) * kBlock + tid;
-                                st_node[slot] = child_base + __popc(imask & ((1u << k) - 1u));
-                                st_t[slot] = tk[k];
+                                const uint32_t off = ring_off(sp + pos);
+                                ring_node(off) = child_base + __popc(imask & ((1u << k) - 1u));
+                                ring_t(off) = tk[k];
                             }
                             // k2 on top, with its key's distance (at most 7
                             // ulps below its entry distance: conservative)
-                            const int slot = ((sp + npush - 1) & (kStack - 1)) * kBlock + tid;
-                            st_node[slot] = child_base + __popc(imask & ((1u << k2) - 1u));
-                            st_t[slot] = __uint_as_float(m2 & ~7u);
+                            const uint32_t off = ring_off(sp + npush - 1);
+                            ring_node(off) = child_base + __popc(imask & ((1u << k2) - 1u));
+                            ring_t(off) = __uint_as_float(m2 & ~7u);
                             sp += npush;
                         }
                     }
