@@ -372,8 +372,8 @@ class HipDevice:
     def set_lookahead(self, enable: bool = True) -> None:
         """Frame lookahead of the fused diffuse renderer (default on): in a
         call that continues the previous one (next pass, same inputs), idle
-        lanes at the end of the frame start the next pass's paths, which the
-        next call keeps if it continues this one.  Images and ray counts per
+        lanes at the end of the frame start the next two passes' paths, which
+        the next call keeps if it continues this one.  Images and ray counts per
         call do not change."""
         self._check(self.lib.prt_hip_set_lookahead(self.ctx, 1 if enable else 0), "set_lookahead")
 

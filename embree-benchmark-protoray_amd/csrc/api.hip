@@ -720,7 +720,7 @@ int prt_hip_render(prt_hip_ctx* ctx, int renderer, int max_depth, float albedo, 
     // camera that moves every frame renders each frame exactly as without
     // it; the work is kept only if that previous call ran the lookahead too.
     const bool look_on = fused && renderer == PRT_RENDERER_DIFFUSE && !fast && !counting && ctx->lookahead != 0 &&
-                         ctx->pass < INT32_MAX - 1;  // the next pass must not overflow
+                         ctx->pass < INT32_MAX - kLookLevels;  // the passes started ahead must not overflow
     prt_hip_ctx::LookKey key{};
     if (look_on) {
         key.scene_gen = ctx->scene_gen;

@@ -266,8 +266,8 @@ int prt_hip_set_tail(prt_hip_ctx* ctx, int mode);
 /* Frame lookahead of the fused diffuse renderer (no counterpart in the
  * reference; default on).  In a call that continues the previous call (pass
  * p + 1 after pass p, with the same scene, camera, frame size, max depth and
- * albedo), idle lanes start paths of the next pass while the last paths of
- * this one run; the call returns when its own frame is complete, and the
+ * albedo), idle lanes start paths of the next two passes while the last
+ * paths of this one run; the call returns when its own frame is complete, and the
  * next call keeps that work if it continues this one (else drops it).  The
  * first call after any change renders as without the lookahead.  The
  * accumulator after each call and its ray count are those of rendering the
