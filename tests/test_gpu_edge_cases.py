@@ -148,3 +148,44 @@ def test_error_paths_leave_the_context_usable(prt, orc, scenes):
         ref = orc.Scene(mesh.idx, mesh.pos).render(cam, 16, 16, spp=1, max_depth=4, albedo=0.8, renderer=1,
                                                    numerics=0)
         assert np.array_equal(d.accum(), ref["accum"])
+
+
+@pytest.mark.parametrize("scale", [1e-3, 1.0, 1e4])
+def test_axis_parallel_and_grazing_rays(dev, orc, scenes, scale):
+    """Rays along the axes, rays with one zero direction component (the
+    traversal clamps that reciprocal to 1e20) and rays that run inside box
+    faces or along box edges, over 27 boxes on an integer grid at three
+    scales (the node box exponents change with the scale).  The child boxes
+    are 8-bit quantized and tested with a rounding slack; every hit record
+    must equal the oracle's brute force (every triangle tested) bit for bit,
+    so a box test that rejects a child it should keep shows up as a miss or
+    a farther hit."""
+    parts = []
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                lo = np.array([2 * i, 2 * j, 2 * k], np.float64) * scale
+                parts.append(scenes.box_tris(tuple(lo), tuple(lo + scale)))
+    mesh = scenes.merge(parts)
+    dev.upload_scene(mesh.idx, mesh.pos)
+    dev.build()
+    # origins on the grid of face planes (0, 0.5, 1, ... 5 units), outside
+    # and inside the boxes, and directions along the axes or with one zero
+    vals = np.arange(-1.0, 6.0, 0.5)
+    g = np.stack(np.meshgrid(vals, vals, [-1.0, 0.0, 2.5], indexing="ij"), -1).reshape(-1, 3)
+    dirs = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
+            (1, 1, 0), (1, -1, 0), (0, 1, 1), (0, 1, -1), (1, 0, 1), (-1, 0, 1), (1, 2, 3)]
+    org, d = [], []
+    for dv in dirs:
+        for o in g:
+            # permute the origin's axes so that every axis sees the grid values
+            for perm in ((0, 1, 2), (2, 0, 1), (1, 2, 0)):
+                org.append(o[list(perm)])
+                d.append(dv)
+    org = np.asarray(org) * scale
+    rays = _rays(org, np.asarray(d, np.float64))
+    hits = dev.intersect(rays)
+    ref = orc.Scene(mesh.idx, mesh.pos).intersect_fp32_brute(rays)
+    bad = np.flatnonzero((hits.view(np.uint32) != ref.view(np.uint32)).any(1))
+    assert bad.size == 0, f"{bad.size} of {len(rays)} rays differ, first {rays[bad[:3]]}"
+    assert (hits[:, 1].view(np.int32) >= 0).sum() > len(rays) // 10
