@@ -548,7 +548,7 @@ def main() -> int:
     cont = rays - paths
     kbytes = fused_bytes(rays, paths, n_node, n_tri)
     achieved = kbytes / (kernel_ms * 1e-3) / 1e9
-    survey_bytes = wavefront_bytes(rays, cont, paths, n_node, n_tri)
+    wavefront_formula_bytes = wavefront_bytes(rays, cont, paths, n_node, n_tri)
     depth_rays = [sum(s["queue"][k] for s in cstats) // args.steps for k in range(depth + 1)]
     wsteps = max(1, sum(s["waveSteps"] for s in cstats))
     lsteps = sum(s["laneSteps"] for s in cstats)
@@ -608,7 +608,7 @@ def main() -> int:
         "device_mrays": round(rays / (dev_ms * 1e-3) / 1e6, 2),
         "no_lookahead_mrays": round(no_look, 2),
         "wavefront_mrays": round(wrays / wwall / 1e6, 2),
-        "frame_hbm_frac_wavefront_formula": round(survey_bytes / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "frame_hbm_frac_wavefront_formula": round(wavefront_formula_bytes / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "nodes_per_ray": round(n_node, 3), "tris_per_ray": round(n_tri, 3), "traversal_occupancy": occupancy,
         "rays_per_frame": rays // args.steps, "rays_per_depth": depth_rays,
         "build": {"buildMs": round(build["buildMs"], 3), "buildMprim": round(build["buildMprim"], 2)},

@@ -18,7 +18,7 @@
 // Intersection: the reference delegates it to Embree 3 / OptiX Prime, which are
 // absent here.  Two restatements:
 //   - FP64: watertight test (Woop, Benthin, Wald, JCGT 2013) in double over a
-//     BVH2, closest (t, primId) pair; the north-star parity reference.
+//     BVH2, closest (t, primId) pair; the reference-semantics parity check.
 //   - kernel numerics: the same test in fp32 with the double fallback for zero
 //     edge functions and a (t, primId) lexicographic minimum, evaluated over
 //     every triangle the double-precision box test admits; the GPU kernel must

@@ -8,7 +8,7 @@ Bars (tolerances written here):
     gradient norm and on every entry above 1e-3 of the largest;
   - adjoint vs central finite differences (h = 1e-3) of the L2 loss against a
     target rendered from a second albedo set, on 256 random triangles that
-    the paths reach: 1e-3 relative (the north-star bar);
+    the paths reach: 1e-3 relative;
   - the torch.autograd.Function returns the same gradient as the C ABI.
 """
 import numpy as np
